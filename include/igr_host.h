@@ -1,0 +1,93 @@
+/* Host library of the MI355X global-routing path (libigr_host.so).
+ *
+ * Pure host code: design generation and the .cap/.net formats, access-point
+ * selection, subnets and Steiner trees, routing inputs, the route file writer
+ * and the evaluator.  Every entry point returns 0 (or a non-null handle) on
+ * success; on failure igrh_error() describes the error of the calling thread.
+ *
+ * Reference counterparts:
+ *   igrh_load      db::read (src/database.hpp:247-415) and db::net::init
+ *                  (src/database.hpp:38-243, run through igrh_build)
+ *   igrh_build     build_cuda_database's host part (src/database_cuda.hpp:
+ *                  2096-2256) and the tree/DFS preparation of
+ *                  Lshape_route (src/Lshape_route.hpp:155-208)
+ *   igrh_write_routes  graph::output_nets (src/graph.hpp:179-235)
+ *   igrh_evaluate  the evaluator program (run/evaluator.cpp:458-633)
+ */
+#ifndef IGR_HOST_H
+#define IGR_HOST_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct igrh_design igrh_design;
+
+typedef struct igrh_gen_params {
+    int layers, xsize, ysize;
+    int64_t nets;
+    int pin_cap;
+    double geom_p, sigma, blockage_frac, heavy_frac;
+    int heavy_min, heavy_max, heavy_window, hotspots;
+    double hotspot_scale;
+    uint64_t seed;
+} igrh_gen_params;
+
+typedef struct igrh_grid_info {
+    int L, X, Y, DIR;              /* routing layers (metal layers - 1) */
+    double unit_wire, unit_via;
+    const double* w;               /* per routing layer overflow weight */
+    const int* xlen;
+    const int* ylen;
+    const float* cap;              /* L*X*Y, index l*X*Y + x*Y + y */
+    int metal_layers;
+    int64_t nets, subnets;
+    int max_nodes;
+} igrh_grid_info;
+
+typedef struct igrh_score {
+    double wirelength, via, overflow, total;
+    int64_t opens;
+} igrh_score;
+
+const char* igrh_error(void);
+
+/* Benchmark configurations 0..4 (nets_override/seed_override 0 = keep). */
+igrh_design* igrh_generate(int preset, int64_t nets_override, uint64_t seed_override);
+igrh_design* igrh_generate_custom(const igrh_gen_params* p);
+igrh_design* igrh_load(const char* cap_path, const char* net_path);
+int igrh_save(const igrh_design* h, const char* cap_path, const char* net_path);
+void igrh_free(igrh_design* h);
+
+/* Access points, routing grid, subnets, trees and routing inputs. */
+int igrh_build(igrh_design* h, int threads);
+double igrh_build_seconds(const igrh_design* h);
+int igrh_grid(const igrh_design* h, igrh_grid_info* out);
+
+/* Named host arrays (node_off, node, parent, pt_off, pt, hs_off, hs, vs_off,
+ * vs, order, hpwl, ...): returns the element count and sets *ptr. */
+int64_t igrh_array(const igrh_design* h, const char* name, const void** ptr);
+
+/* Routes per subnet: off[s] indexes data; data[off[s]] = next free slot,
+ * then cell index pairs (routing layers). */
+int igrh_write_routes(const igrh_design* h, const char* out_path, const int64_t* off, const int* data,
+                      int64_t data_len);
+int igrh_evaluate(const igrh_design* h, const char* out_path, igrh_score* out);
+
+#ifdef __cplusplus
+}
+namespace igr {
+struct Design;
+struct NetPins;
+struct Grid;
+struct Subnets;
+struct PatternInputs;
+}
+const igr::Design* igrh_design_ref(const igrh_design* h);
+const igr::NetPins* igrh_netpins_ref(const igrh_design* h);
+const igr::Grid* igrh_grid_ref(const igrh_design* h);
+const igr::Subnets* igrh_subnets_ref(const igrh_design* h);
+const igr::PatternInputs* igrh_inputs_ref(const igrh_design* h);
+#endif
+#endif

@@ -1,0 +1,150 @@
+// C interface of the host library (libigr_host.so): design generation and
+// I/O, access points, subnets and trees, routing inputs, route writing and the
+// evaluator.  Used by the C++ driver, the GPU library and the Python tests.
+#include "igr_host.h"
+#include "prep.hpp"
+
+#include <chrono>
+#include <cstring>
+#include <exception>
+#include <string>
+
+using namespace igr;
+
+struct igrh_design {
+    Design d;
+    NetPins np;
+    Grid g;
+    Subnets s;
+    PatternInputs pi;
+    bool built = false;
+    double t_tree = 0;
+};
+
+static thread_local std::string g_err;
+const char* igrh_error(void) { return g_err.c_str(); }
+
+template <typename F>
+static int guard(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+igrh_design* igrh_generate(int preset, int64_t nets_override, uint64_t seed_override) {
+    igrh_design* h = new igrh_design();
+    if (guard([&]() {
+            GenConfig c = preset_config(preset);
+            if (nets_override > 0) c.nets = nets_override;
+            if (seed_override) c.seed = seed_override;
+            generate_design(c, h->d);
+        })) { delete h; return nullptr; }
+    return h;
+}
+
+igrh_design* igrh_generate_custom(const igrh_gen_params* p) {
+    igrh_design* h = new igrh_design();
+    if (guard([&]() {
+            GenConfig c;
+            c.L = p->layers; c.X = p->xsize; c.Y = p->ysize; c.nets = p->nets;
+            c.pin_cap = p->pin_cap; c.geom_p = p->geom_p; c.sigma = p->sigma;
+            c.blockage_frac = p->blockage_frac; c.heavy_frac = p->heavy_frac;
+            c.heavy_min = p->heavy_min; c.heavy_max = p->heavy_max; c.heavy_window = p->heavy_window;
+            c.hotspots = p->hotspots; c.hotspot_scale = p->hotspot_scale; c.seed = p->seed;
+            generate_design(c, h->d);
+        })) { delete h; return nullptr; }
+    return h;
+}
+
+igrh_design* igrh_load(const char* cap_path, const char* net_path) {
+    igrh_design* h = new igrh_design();
+    if (guard([&]() {
+            read_cap(cap_path, h->d);
+            read_net(net_path, h->d);
+        })) { delete h; return nullptr; }
+    return h;
+}
+
+int igrh_save(const igrh_design* h, const char* cap_path, const char* net_path) {
+    return guard([&]() {
+        write_cap(cap_path, h->d);
+        write_net(net_path, h->d);
+    });
+}
+
+void igrh_free(igrh_design* h) { delete h; }
+
+int igrh_build(igrh_design* h, int threads) {
+    return guard([&]() {
+        auto t0 = std::chrono::steady_clock::now();
+        select_access_points(h->d, h->np, threads);
+        build_grid(h->d, h->g);
+        build_subnets(h->d, h->np, h->g, h->s, threads);
+        prepare_pattern_inputs(h->g, h->s, h->pi, threads);
+        h->t_tree = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        h->built = true;
+    });
+}
+
+double igrh_build_seconds(const igrh_design* h) { return h->t_tree; }
+
+int igrh_grid(const igrh_design* h, igrh_grid_info* out) {
+    if (!h->built) { g_err = "design not built"; return -1; }
+    const Grid& g = h->g;
+    out->L = g.L; out->X = g.X; out->Y = g.Y; out->DIR = g.DIR;
+    out->unit_wire = g.wire_cost; out->unit_via = g.via_cost;
+    out->w = g.w.data(); out->xlen = g.xlen.data(); out->ylen = g.ylen.data(); out->cap = g.cap.data();
+    out->metal_layers = h->d.L;
+    out->nets = h->d.num_nets();
+    out->subnets = h->s.count;
+    out->max_nodes = h->pi.max_nodes;
+    return 0;
+}
+
+int64_t igrh_array(const igrh_design* h, const char* name, const void** ptr) {
+    const PatternInputs& pi = h->pi;
+    const Subnets& s = h->s;
+    std::string n(name);
+#define A(key, vec) if (n == key) { *ptr = (vec).data(); return (int64_t)(vec).size(); }
+    A("node_off", pi.node_off) A("node", pi.node) A("parent", pi.parent)
+    A("pt_off", pi.pt_off) A("pt", pi.pt) A("hs_off", pi.hs_off) A("hs", pi.hs)
+    A("vs_off", pi.vs_off) A("vs", pi.vs) A("order", pi.order)
+    A("hpwl", s.hpwl) A("orig", s.orig) A("sub_pin_off", s.pin_off) A("sub_pins", s.pins)
+    A("tree_node_off", s.node_off) A("tree_node", s.node) A("tree_adj_off", s.adj_off) A("tree_adj", s.adj)
+    A("net_sub_off", s.net_sub_off) A("net_sub", s.net_sub)
+    A("pin_off", h->np.pin_off) A("pins", h->np.pins) A("extra_off", h->np.extra_off) A("extra", h->np.extra)
+    A("ap_off", h->d.ap_off) A("ap", h->d.ap) A("net_pin_off", h->d.pin_off)
+#undef A
+    g_err = "unknown array " + n;
+    *ptr = nullptr;
+    return -1;
+}
+
+int igrh_write_routes(const igrh_design* h, const char* out_path, const int64_t* off, const int* data,
+                      int64_t data_len) {
+    return guard([&]() {
+        RouteSet rs;
+        rs.off.assign(off, off + h->s.count + 1);
+        rs.data.assign(data, data + data_len);
+        write_routes(out_path, h->d, h->np, h->g, h->s, rs);
+    });
+}
+
+int igrh_evaluate(const igrh_design* h, const char* out_path, igrh_score* out) {
+    return guard([&]() {
+        Score sc;
+        evaluate_routes(h->d, out_path, sc);
+        out->wirelength = sc.wl; out->via = sc.via; out->overflow = sc.of; out->total = sc.total;
+        out->opens = sc.opens;
+    });
+}
+
+const igr::Design* igrh_design_ref(const igrh_design* h) { return &h->d; }
+const igr::NetPins* igrh_netpins_ref(const igrh_design* h) { return &h->np; }
+const igr::Grid* igrh_grid_ref(const igrh_design* h) { return &h->g; }
+const igr::Subnets* igrh_subnets_ref(const igrh_design* h) { return &h->s; }
+const igr::PatternInputs* igrh_inputs_ref(const igrh_design* h) { return &h->pi; }

@@ -1,0 +1,182 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_DIR = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+
+
+class OrcGrid(C.Structure):
+    _fields_ = [("L", C.c_int), ("X", C.c_int), ("Y", C.c_int), ("DIR", C.c_int),
+                ("unit_wire", C.c_double), ("unit_via", C.c_double),
+                ("w", C.c_void_p), ("xlen", C.c_void_p), ("ylen", C.c_void_p), ("cap", C.c_void_p)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(_DIR, "_build", "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built (make -C oracle)")
+        L = C.CDLL(path)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.POINTER(OrcGrid)]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_expf_scalar.restype = C.c_float
+        L.orc_expf_scalar.argtypes = [C.c_float]
+        L.orc_expf_many.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+        for f in ("orc_set_demand", "orc_get_demand", "orc_get_timestamp"):
+            getattr(L, f).argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_update_cost.argtypes = [C.c_void_p]
+        L.orc_compute_presum.argtypes = [C.c_void_p]
+        L.orc_get_costs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_set_nets.argtypes = [C.c_void_p, C.c_int64] + [C.c_void_p] * 9
+        L.orc_schedule.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p]
+        L.orc_lshape_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+        L.orc_pattern_route.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+        L.orc_get_route.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+        L.orc_stamp.argtypes = [C.c_void_p]
+        L.orc_ripup_mark.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_congestion_view.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_lshape_probe.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+        L.orc_lshape_probe.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data if a is not None else None
+
+
+def expf(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty_like(x)
+    lib().orc_expf_many(x.ctypes.data, out.ctypes.data, x.size)
+    return out
+
+
+class Oracle:
+    """Sequential restatement of the reference routing logic on one grid."""
+
+    def __init__(self, L, X, Y, DIR, unit_wire, unit_via, w, xlen, ylen, cap):
+        self._keep = [np.ascontiguousarray(w, dtype=np.float64), np.ascontiguousarray(xlen, dtype=np.int32),
+                      np.ascontiguousarray(ylen, dtype=np.int32), np.ascontiguousarray(cap, dtype=np.float32)]
+        g = OrcGrid(L, X, Y, DIR, unit_wire, unit_via, *[k.ctypes.data for k in self._keep])
+        self.L, self.X, self.Y, self.DIR = L, X, Y, DIR
+        self.n = L * X * Y
+        self.h = C.c_void_p(lib().orc_create(C.byref(g)))
+
+    @classmethod
+    def from_design(cls, d):
+        i = d.info
+        cap, w, xlen, ylen = d.grid_arrays()
+        return cls(i.L, i.X, i.Y, i.DIR, i.unit_wire, i.unit_via, w, xlen, ylen, cap)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_destroy(self.h)
+            self.h = None
+
+    def set_demand(self, d):
+        d = np.ascontiguousarray(d, dtype=np.float32)
+        lib().orc_set_demand(self.h, d.ctypes.data)
+
+    def demand(self):
+        out = np.empty(self.n, np.float32)
+        lib().orc_get_demand(self.h, out.ctypes.data)
+        return out
+
+    def timestamp(self):
+        out = np.empty(self.n, np.int32)
+        lib().orc_get_timestamp(self.h, out.ctypes.data)
+        return out
+
+    def update_cost(self):
+        lib().orc_update_cost(self.h)
+
+    def compute_presum(self):
+        lib().orc_compute_presum(self.h)
+
+    def costs(self):
+        wc = np.empty(self.n, np.float32)
+        vc = np.empty(self.n, np.float32)
+        ps = np.empty(self.n, np.float64)
+        lib().orc_get_costs(self.h, wc.ctypes.data, vc.ctypes.data, ps.ctypes.data)
+        return wc, vc, ps
+
+    def set_nets_from_design(self, d):
+        a = d.array
+        self._nets = [a("node_off"), a("node"), a("parent"), a("pt_off"), a("pt"), a("hs_off"), a("hs"),
+                      a("vs_off"), a("vs")]
+        n = len(self._nets[0]) - 1
+        rc = lib().orc_set_nets(self.h, n, *[_p(x) for x in self._nets])
+        if rc != 0:
+            raise RuntimeError("bad tree order")
+        self.num_nets = n
+
+    def set_nets(self, node_off, node, parent, pt_off=None, pt=None, hs_off=None, hs=None, vs_off=None, vs=None):
+        arrs = [np.ascontiguousarray(node_off, np.int64), np.ascontiguousarray(node, np.int32),
+                np.ascontiguousarray(parent, np.int32)]
+        for off, v in ((pt_off, pt), (hs_off, hs), (vs_off, vs)):
+            arrs += [None if off is None else np.ascontiguousarray(off, np.int64),
+                     None if v is None else np.ascontiguousarray(v, np.int32)]
+        self._nets = arrs
+        n = len(arrs[0]) - 1
+        rc = lib().orc_set_nets(self.h, n, *[_p(x) for x in arrs])
+        if rc != 0:
+            raise RuntimeError("bad tree order")
+        self.num_nets = n
+
+    def schedule(self, order, mode=0):
+        order = np.ascontiguousarray(order, np.int32)
+        out = np.empty(len(order), np.int32)
+        nb = lib().orc_schedule(self.h, order.ctypes.data, len(order), mode, out.ctypes.data)
+        return nb, out
+
+    def lshape_batch(self, nets):
+        nets = np.ascontiguousarray(nets, np.int32)
+        lib().orc_lshape_batch(self.h, nets.ctypes.data, len(nets))
+
+    def pattern_route(self, order):
+        order = np.ascontiguousarray(order, np.int32)
+        return lib().orc_pattern_route(self.h, order.ctypes.data, len(order))
+
+    def route(self, net):
+        buf = np.empty(4096, np.int32)
+        n = lib().orc_get_route(self.h, net, buf.ctypes.data, buf.size)
+        if n > buf.size:
+            buf = np.empty(n, np.int32)
+            lib().orc_get_route(self.h, net, buf.ctypes.data, buf.size)
+        return buf[:n].copy()
+
+    def routes_flat(self):
+        rs = [self.route(i) for i in range(self.num_nets)]
+        off = np.zeros(len(rs) + 1, np.int64)
+        off[1:] = np.cumsum([len(r) for r in rs])
+        return off, (np.concatenate(rs) if rs else np.zeros(0, np.int32))
+
+    def ripup_mark(self, th=0):
+        out = np.empty(self.num_nets, np.uint8)
+        lib().orc_ripup_mark(self.h, th, out.ctypes.data)
+        return out
+
+    def congestion_view(self):
+        xs = np.empty(self.X * self.Y, np.float32)
+        ys = np.empty(self.X * self.Y, np.float32)
+        lib().orc_congestion_view(self.h, xs.ctypes.data, ys.ctypes.data)
+        return xs, ys
+
+    def lshape_probe(self, net):
+        buf = np.empty(8192, np.int32)
+        best = lib().orc_lshape_probe(self.h, net, buf.ctypes.data, buf.size)
+        return best, buf[:buf[0]].copy()
+
+    @property
+    def stamp(self):
+        return lib().orc_stamp(self.h)
