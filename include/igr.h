@@ -1,0 +1,116 @@
+/* C ABI of the MI355X global-routing path (libigr.so).
+ *
+ * The reference exposes no library interface: its routing stage is a set of
+ * C++ functions over process-global __managed__ state, called from route()
+ * (src/main.cpp:22-73).  This ABI replaces that internal boundary with an
+ * explicit context on one GPU.  Conventions: inputs are host pointers that are
+ * copied (the caller keeps ownership), outputs go to caller-allocated host
+ * buffers, every call returns IGR_OK (0) or a negative status and never exits
+ * the process, igr_last_error() describes the last failure of the calling
+ * thread, and one host thread drives a context at a time.
+ *
+ * Grid arrays use the reference cell index l*X*Y + x*Y + y over routing layers
+ * (metal layer - 1), see src/database_cuda.hpp:49 and :2106.
+ */
+#ifndef IGR_H
+#define IGR_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { IGR_OK = 0, IGR_EINVAL = -1, IGR_EHIP = -2, IGR_ESTATE = -3 };
+enum { IGR_FORM_PLAIN = 0, IGR_FORM_BATCHED = 1 };
+
+typedef struct igr_ctx igr_ctx;
+
+typedef struct igr_grid_desc {
+    int L, X, Y, DIR;            /* routing layers, gcells, direction of routing layer 0 (0 = horizontal) */
+    double unit_wire, unit_via;
+    const double* of_weight;     /* L overflow weights */
+    const int* xlen;             /* X-1 gcell edge lengths */
+    const int* ylen;             /* Y-1 */
+    const float* capacity;       /* L*X*Y */
+} igr_grid_desc;
+
+/* Subnet trees in DFS pre-order (node 0 is the root, a parent precedes its
+ * children); node = l*X*Y + x*Y + y with l = L for Steiner nodes.  Scheduler
+ * footprints: points as x*Y+y, horizontal/vertical segments as pairs of
+ * x*Y+y.  Offsets have n+1 entries; segment offsets count segments. */
+typedef struct igr_net_desc {
+    int64_t n;
+    const int64_t* node_off;
+    const int* nodes;
+    const int* parents;
+    const int64_t* pt_off;
+    const int* pts;
+    const int64_t* hs_off;
+    const int* hs;
+    const int64_t* vs_off;
+    const int* vs;
+} igr_net_desc;
+
+typedef struct igr_pass_stats {
+    int batches;
+    double schedule_s;            /* scheduler wall time */
+    double route_s;               /* batch routing wall time (device work included) */
+    int64_t nets;
+} igr_pass_stats;
+
+/* Context lifetime.  Replaces build_cuda_database's grid part
+ * (src/database_cuda.hpp:2258-2361). */
+int igr_create(const igr_grid_desc* desc, igr_ctx** out);
+void igr_destroy(igr_ctx* ctx);
+const char* igr_last_error(void);
+
+/* Grid state (reference layout on the host side). */
+int igr_set_demand(igr_ctx* ctx, const float* demand);
+int igr_get_demand(igr_ctx* ctx, float* demand);
+int igr_get_timestamp(igr_ctx* ctx, int* ts);
+int igr_get_costs(igr_ctx* ctx, float* wcost, float* vcost, double* presum);  /* any may be NULL */
+
+/* graph::update_cost (src/graph.hpp:140-146) */
+int igr_update_cost(igr_ctx* ctx);
+/* graph::compute_presum (src/graph.hpp:405-430) */
+int igr_compute_presum(igr_ctx* ctx);
+
+/* Nets of the pattern pass (the tree/DFS upload of src/Lshape_route.hpp:184-229). */
+int igr_set_nets(igr_ctx* ctx, const igr_net_desc* nets);
+
+/* generate_batches_rsmt_gpu (src/database_cuda.hpp:437-631): order is the
+ * routing priority (highest first); batch_of[i] receives the batch of
+ * order[i]; *nbatches the batch count. */
+int igr_schedule(igr_ctx* ctx, const int* order, int64_t n, int mode, int* batch_of, int* nbatches);
+
+/* One pattern-routing batch (src/Lshape_route.hpp:235-242): cost update,
+ * prefix sums, L-shape DP of every listed net, wire and via commit. */
+int igr_route_batch(igr_ctx* ctx, const int* nets, int64_t n, int form);
+
+/* Lshape_route::Lshape_route (src/Lshape_route.hpp:155-257) after the trees:
+ * schedule, then route the batches in the reference order. */
+int igr_pattern_route(igr_ctx* ctx, const int* order, int64_t n, int form, igr_pass_stats* stats);
+
+/* Routes: r[0] = next free slot (1 = empty), then pairs of cell indices. */
+int igr_get_route(igr_ctx* ctx, int net, int* buf, int cap, int* len);
+/* All routes packed: off[n+1] (host), data with data_cap entries. */
+int igr_get_routes(igr_ctx* ctx, int64_t* off, int* data, int64_t data_cap);
+int64_t igr_routes_capacity(igr_ctx* ctx);
+
+/* graph::ripup's marking (src/graph.hpp:588-601): is_of[net] = 1 when more
+ * than 10 of the net's wire edges have demand >= capacity + threshold. */
+int igr_ripup_mark(igr_ctx* ctx, int threshold, uint8_t* is_of);
+/* graph::extract_congestionView{,_xsum,_ysum} (src/graph.hpp:545-584):
+ * X*Y prefix views indexed x*Y + y. */
+int igr_congestion_view(igr_ctx* ctx, float* xsum, float* ysum);
+
+/* Current commit timestamp counter. */
+int igr_stamp(igr_ctx* ctx);
+
+/* The routing exp evaluated on the device (diagnostic). */
+int igr_expf(const float* in, float* out, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

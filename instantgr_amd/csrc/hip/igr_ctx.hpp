@@ -1,0 +1,107 @@
+// Routing context: device grid state, net data and the launch helpers shared
+// by the kernel translation units.
+#pragma once
+#include "igr_dev.h"
+#include "igr.h"
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#define HIP_OK(expr)                                                                               \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+namespace igr_dev {
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count <= n && p) return;
+        release();
+        if (count == 0) count = 1;
+        HIP_OK(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        alloc(count);
+        if (count) HIP_OK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void zero(size_t count, hipStream_t s) {
+        alloc(count);
+        HIP_OK(hipMemsetAsync(p, 0, count * sizeof(T), s));
+    }
+};
+
+// Net trees and footprints on the device (pattern pass).
+struct NetsDev {
+    int64_t n = 0;
+    DevBuf<int64_t> node_off, route_off, pt_off, hs_off, vs_off;
+    DevBuf<int> node, parent, routes, pt, hs, vs;
+    std::vector<int64_t> h_node_off, h_route_off;
+    std::vector<int> h_node;          // host copies used by the host-side batch logic
+    int max_nodes = 0;
+};
+
+struct Sched {
+    DevBuf<int> map, batch, status, viable, ids, flags;
+};
+
+struct Scratch {
+    DevBuf<double> dist;
+    DevBuf<int> from, lr;
+    DevBuf<int> net_list;
+    DevBuf<int64_t> scr_off;
+    DevBuf<int> of;
+    DevBuf<uint8_t> is_of;
+    DevBuf<float> xsum, ysum;
+};
+
+} // namespace igr_dev
+
+struct igr_ctx {
+    igr_dev::GridDev g{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    igr_dev::DevBuf<double> w;
+    igr_dev::DevBuf<int> xlen, ylen;
+    igr_dev::DevBuf<float> cap, demand, wcost, vcost;
+    igr_dev::DevBuf<double> presum;
+    igr_dev::DevBuf<int> pre, ts;
+    igr_dev::NetsDev nets;
+    igr_dev::Sched sched;
+    igr_dev::Scratch scratch;
+    int stamp = 0;
+    int64_t cells() const { return (int64_t)g.L * g.X * g.Y; }
+};
+
+namespace igr_dev {
+// Kernel launchers (defined in the .hip files).
+void launch_update_cost(igr_ctx* c);
+void launch_presum(igr_ctx* c);
+void launch_wire_commit(igr_ctx* c, int stamp);
+void launch_via_commit(igr_ctx* c, const int* d_nets, int64_t n, int stamp, int K);
+void launch_lshape_plain(igr_ctx* c, const int* d_nets, const int64_t* d_scr_off, int64_t n, int stamp);
+void launch_to_ref(igr_ctx* c, const void* src, void* dst, int elem_bytes);
+void launch_from_ref(igr_ctx* c, const void* src, void* dst, int elem_bytes);
+void launch_ripup_mark(igr_ctx* c, int th, uint8_t* d_is_of, int* d_of);
+void launch_congestion(igr_ctx* c, float* d_xsum, float* d_ysum);
+int run_schedule(igr_ctx* c, const int* h_order, int64_t n, int mode, int* h_batch_of);
+void launch_expf(const float* in, float* out, int64_t n, hipStream_t s);
+} // namespace igr_dev

@@ -1,0 +1,97 @@
+// Device-side definitions shared by the routing kernels: grid layout, the
+// routing exp, the logistic cost and small helpers.
+//
+// Grid layout in HBM (structure of arrays, one array per quantity):
+// routing layer l occupies [l*X*Y, (l+1)*X*Y).  Inside a layer the cells are
+// stored track-major: a vertical layer (dir 1, tracks along y) keeps
+// x*Y + y, a horizontal layer (dir 0, tracks along x) keeps y*X + x, so every
+// track (row of a horizontal layer, column of a vertical layer) is contiguous
+// and the per-track scans and commits read and write whole cache lines.
+// Routes and every host-visible array keep the reference encoding
+// l*X*Y + x*Y + y (src/database_cuda.hpp:49).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace igr_dev {
+
+constexpr double INF = 1e22;      // src/database_cuda.hpp:53
+constexpr int MAXL = 16;          // largest supported routing-layer count
+
+struct GridDev {
+    int L, X, Y, DIR;
+    int64_t XYc;                  // X*Y
+    double unit_wire, unit_via;
+    const double* w;              // per layer overflow weight (device)
+    const int* xlen;
+    const int* ylen;
+    const float* cap;
+    float* demand;
+    float* wcost;
+    float* vcost;
+    double* presum;
+    int* pre;                     // wire demand differences
+    int* ts;                      // commit timestamps
+};
+
+__host__ __device__ __forceinline__ int layer_dir(int l, int DIR) { return (l & 1) ^ DIR; }
+
+// Device offset of cell (l, x, y).
+__host__ __device__ __forceinline__ int64_t gidx(const GridDev& g, int l, int x, int y) {
+    return (int64_t)l * g.XYc + (layer_dir(l, g.DIR) ? (int64_t)x * g.Y + y : (int64_t)y * g.X + x);
+}
+
+// ---------------------------------------------------------------------------
+// Routing exp: Cody-Waite reduction and a degree-7 Taylor polynomial with fused
+// multiply-adds; +inf above 88.72, 0 below -87, NaN propagates.  The CPU
+// oracle evaluates the identical operation sequence, so both sides agree bit
+// for bit.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float pow2i(int k) { return __int_as_float((k + 127) << 23); }
+
+__device__ __forceinline__ float igr_expf(float x) {
+    if (x != x) return x;
+    if (x > 88.72f) return __int_as_float(0x7f800000);
+    if (x < -87.0f) return 0.0f;
+    const float log2e = 1.44269504088896341f;
+    const float ln2_hi = 0.693145751953125f;
+    const float ln2_lo = 1.42860676533018704e-06f;
+    float n = __builtin_rintf(x * log2e);
+    float r = __builtin_fmaf(-n, ln2_hi, x);
+    r = __builtin_fmaf(-n, ln2_lo, r);
+    float p = 1.98412698412698413e-04f;
+    p = __builtin_fmaf(p, r, 1.38888888888888889e-03f);
+    p = __builtin_fmaf(p, r, 8.33333333333333333e-03f);
+    p = __builtin_fmaf(p, r, 4.16666666666666667e-02f);
+    p = __builtin_fmaf(p, r, 1.66666666666666667e-01f);
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    int k = (int)n;
+    int k1 = k / 2, k2 = k - k1;
+    return (p * pow2i(k1)) * pow2i(k2);
+}
+
+// Logistic overflow cost of one edge (src/graph.hpp:23-32 with
+// of_cost_scale = 1): e^(0.5 (d - c)) for c > 0.001, e^(1.5 d) for d > 0.
+__device__ __forceinline__ double of_cost(float c, float d) {
+    if ((double)c > 0.001) {
+        double a = 0.5 * (double)(d - c);
+        return (double)igr_expf((float)a);
+    }
+    if (d > 0) {
+        double a = 1.5 * (double)d;
+        return (double)igr_expf((float)a);
+    }
+    return 0;
+}
+
+// Cost of adding 'inc' tracks of demand to an edge (src/graph.hpp:34-38):
+// min(1e12, (f(d + inc) - f(d)) * w) with the min absorbing inf - inf.
+__device__ __forceinline__ double inc_cost(float c, float d, double inc, double w) {
+    float d1 = (float)((double)d + inc + 0.0);
+    double v = (of_cost(c, d1) - of_cost(c, d)) * w;
+    return fmin(1e12, v);
+}
+
+} // namespace igr_dev

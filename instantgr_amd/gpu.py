@@ -1,0 +1,206 @@
+"""ctypes binding of libigr.so, the HIP routing library.
+
+Raises at load time when the library is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+_lib = None
+
+FORM_PLAIN = 0
+FORM_BATCHED = 1
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("L", C.c_int), ("X", C.c_int), ("Y", C.c_int), ("DIR", C.c_int),
+                ("unit_wire", C.c_double), ("unit_via", C.c_double),
+                ("of_weight", C.c_void_p), ("xlen", C.c_void_p), ("ylen", C.c_void_p), ("capacity", C.c_void_p)]
+
+
+class NetDesc(C.Structure):
+    _fields_ = [("n", C.c_int64)] + [(k, C.c_void_p) for k in
+                                      ("node_off", "nodes", "parents", "pt_off", "pts", "hs_off", "hs", "vs_off", "vs")]
+
+
+class PassStats(C.Structure):
+    _fields_ = [("batches", C.c_int), ("schedule_s", C.c_double), ("route_s", C.c_double), ("nets", C.c_int64)]
+
+
+# Every entry point of include/igr.h with its ctypes signature.
+SIGNATURES = {
+    "igr_create": (C.c_int, [C.POINTER(GridDesc), C.POINTER(C.c_void_p)]),
+    "igr_destroy": (None, [C.c_void_p]),
+    "igr_last_error": (C.c_char_p, []),
+    "igr_set_demand": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "igr_get_demand": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "igr_get_timestamp": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "igr_get_costs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "igr_update_cost": (C.c_int, [C.c_void_p]),
+    "igr_compute_presum": (C.c_int, [C.c_void_p]),
+    "igr_set_nets": (C.c_int, [C.c_void_p, C.POINTER(NetDesc)]),
+    "igr_schedule": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.POINTER(C.c_int)]),
+    "igr_route_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int]),
+    "igr_pattern_route": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.POINTER(PassStats)]),
+    "igr_get_route": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "igr_get_routes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
+    "igr_routes_capacity": (C.c_int64, [C.c_void_p]),
+    "igr_ripup_mark": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "igr_congestion_view": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "igr_stamp": (C.c_int, [C.c_void_p]),
+    "igr_expf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+}
+
+
+def lib_path():
+    return os.path.join(_LIB_DIR, "libigr.so")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError("libigr.so is not built; run __graft_entry__.build()")
+        L = C.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError("libigr: %s (status %d)" % (lib().igr_last_error().decode(), rc))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def expf(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty_like(x)
+    _check(lib().igr_expf(x.ctypes.data, out.ctypes.data, x.size))
+    return out
+
+
+class Router:
+    """One routing context on the current GPU (grid state + nets)."""
+
+    def __init__(self, L, X, Y, DIR, unit_wire, unit_via, w, xlen, ylen, cap):
+        self._keep = [np.ascontiguousarray(w, np.float64), np.ascontiguousarray(xlen, np.int32),
+                      np.ascontiguousarray(ylen, np.int32), np.ascontiguousarray(cap, np.float32)]
+        d = GridDesc(L, X, Y, DIR, unit_wire, unit_via, *[k.ctypes.data for k in self._keep])
+        h = C.c_void_p()
+        _check(lib().igr_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self.L, self.X, self.Y, self.DIR = L, X, Y, DIR
+        self.n = L * X * Y
+        self.num_nets = 0
+
+    @classmethod
+    def from_design(cls, d):
+        i = d.info
+        cap, w, xlen, ylen = d.grid_arrays()
+        return cls(i.L, i.X, i.Y, i.DIR, i.unit_wire, i.unit_via, w, xlen, ylen, cap)
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.igr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_demand(self, d):
+        d = np.ascontiguousarray(d, np.float32)
+        _check(lib().igr_set_demand(self.h, d.ctypes.data))
+
+    def demand(self):
+        out = np.empty(self.n, np.float32)
+        _check(lib().igr_get_demand(self.h, out.ctypes.data))
+        return out
+
+    def timestamp(self):
+        out = np.empty(self.n, np.int32)
+        _check(lib().igr_get_timestamp(self.h, out.ctypes.data))
+        return out
+
+    def costs(self):
+        wc = np.empty(self.n, np.float32)
+        vc = np.empty(self.n, np.float32)
+        ps = np.empty(self.n, np.float64)
+        _check(lib().igr_get_costs(self.h, wc.ctypes.data, vc.ctypes.data, ps.ctypes.data))
+        return wc, vc, ps
+
+    def update_cost(self):
+        _check(lib().igr_update_cost(self.h))
+
+    def compute_presum(self):
+        _check(lib().igr_compute_presum(self.h))
+
+    def set_nets(self, node_off, node, parent, pt_off=None, pt=None, hs_off=None, hs=None, vs_off=None, vs=None):
+        arrs = [np.ascontiguousarray(node_off, np.int64), np.ascontiguousarray(node, np.int32),
+                np.ascontiguousarray(parent, np.int32)]
+        for off, v in ((pt_off, pt), (hs_off, hs), (vs_off, vs)):
+            arrs += [None if off is None else np.ascontiguousarray(off, np.int64),
+                     None if v is None else np.ascontiguousarray(v, np.int32)]
+        self._nets = arrs
+        d = NetDesc(len(arrs[0]) - 1, *[_ptr(a) for a in arrs])
+        _check(lib().igr_set_nets(self.h, C.byref(d)))
+        self.num_nets = len(arrs[0]) - 1
+
+    def set_nets_from_design(self, d):
+        a = d.array
+        self.set_nets(a("node_off"), a("node"), a("parent"), a("pt_off"), a("pt"), a("hs_off"), a("hs"),
+                      a("vs_off"), a("vs"))
+
+    def schedule(self, order, mode=0):
+        order = np.ascontiguousarray(order, np.int32)
+        out = np.empty(len(order), np.int32)
+        nb = C.c_int()
+        _check(lib().igr_schedule(self.h, order.ctypes.data, len(order), mode, out.ctypes.data, C.byref(nb)))
+        return nb.value, out
+
+    def route_batch(self, nets, form=FORM_PLAIN):
+        nets = np.ascontiguousarray(nets, np.int32)
+        _check(lib().igr_route_batch(self.h, nets.ctypes.data, len(nets), form))
+
+    def pattern_route(self, order, form=FORM_PLAIN):
+        order = np.ascontiguousarray(order, np.int32)
+        st = PassStats()
+        _check(lib().igr_pattern_route(self.h, order.ctypes.data, len(order), form, C.byref(st)))
+        return {"batches": st.batches, "schedule_s": st.schedule_s, "route_s": st.route_s, "nets": st.nets}
+
+    def route(self, net):
+        buf = np.empty(65536, np.int32)
+        ln = C.c_int()
+        _check(lib().igr_get_route(self.h, net, buf.ctypes.data, buf.size, C.byref(ln)))
+        return buf[:ln.value].copy()
+
+    def routes_flat(self):
+        cap = lib().igr_routes_capacity(self.h)
+        off = np.empty(self.num_nets + 1, np.int64)
+        data = np.empty(max(cap, 1), np.int32)
+        _check(lib().igr_get_routes(self.h, off.ctypes.data, data.ctypes.data, data.size))
+        return off, data
+
+    def ripup_mark(self, th=0):
+        out = np.empty(self.num_nets, np.uint8)
+        _check(lib().igr_ripup_mark(self.h, th, out.ctypes.data))
+        return out
+
+    def congestion_view(self):
+        xs = np.empty(self.X * self.Y, np.float32)
+        ys = np.empty(self.X * self.Y, np.float32)
+        _check(lib().igr_congestion_view(self.h, xs.ctypes.data, ys.ctypes.data))
+        return xs, ys
+
+    @property
+    def stamp(self):
+        return lib().igr_stamp(self.h)

@@ -1,0 +1,127 @@
+"""GPU parity: every kernel of the pattern pass against the CPU oracle,
+bit for bit, through the C ABI of libigr.so."""
+import numpy as np
+import pytest
+
+import orc
+from instantgr_amd import gpu
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_demand(n, seed, big=True):
+    rng = np.random.default_rng(seed)
+    d = rng.integers(0, 24, size=n).astype(np.float32) * 0.5
+    if big:
+        idx = rng.choice(n, size=min(n, 64), replace=False)
+        d[idx] = rng.choice([40.0, 59.5, 60.0, 75.0, 120.0, 400.0], size=len(idx)).astype(np.float32)
+    return d
+
+
+def test_expf_bitexact():
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([
+        np.linspace(-100, 100, 2_000_001, dtype=np.float32),
+        rng.uniform(-90, 90, 1_000_000).astype(np.float32),
+        np.array([0.0, -0.0, 88.72, 88.7201, -87.0, -87.0001, np.inf, -np.inf, np.nan], np.float32),
+    ])
+    a = gpu.expf(xs)
+    b = orc.expf(xs)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_cost_update_and_presum_bitexact(design_c1):
+    d = design_c1
+    r = gpu.Router.from_design(d)
+    o = orc.Oracle.from_design(d)
+    dem = _random_demand(r.n, 1)
+    r.set_demand(dem)
+    o.set_demand(dem)
+    r.update_cost()
+    o.update_cost()
+    r.compute_presum()
+    o.compute_presum()
+    wg, vg, pg = r.costs()
+    wo, vo, po = o.costs()
+    assert np.array_equal(wg.view(np.uint32), wo.view(np.uint32))
+    assert np.array_equal(vg.view(np.uint32), vo.view(np.uint32))
+    assert np.array_equal(pg.view(np.uint64), po.view(np.uint64))
+    r.close()
+
+
+@pytest.mark.parametrize("shape", [(4, 37, 300), (5, 300, 9), (3, 1000, 7), (2, 2500, 3)])
+def test_presum_track_lengths(shape):
+    """Tracks from 7 to 2500 gcells: every chunking of the workgroup scan."""
+    L, X, Y = shape
+    rng = np.random.default_rng(X + Y)
+    cap = rng.integers(0, 7, size=L * X * Y).astype(np.float32)
+    w = np.ones(L)
+    xlen = rng.integers(1, 5, size=X - 1)
+    ylen = rng.integers(1, 5, size=Y - 1)
+    r = gpu.Router(L, X, Y, 1, 1.0, 4.0, w, xlen, ylen, cap)
+    o = orc.Oracle(L, X, Y, 1, 1.0, 4.0, w, xlen, ylen, cap)
+    dem = _random_demand(r.n, X, big=False)
+    for h in (r, o):
+        h.set_demand(dem)
+        h.update_cost()
+        h.compute_presum()
+    for a, b in zip(r.costs(), o.costs()):
+        assert np.array_equal(a, b)
+    r.close()
+
+
+def test_schedule_bitexact(design_c1):
+    d = design_c1
+    r = gpu.Router.from_design(d)
+    o = orc.Oracle.from_design(d)
+    r.set_nets_from_design(d)
+    o.set_nets_from_design(d)
+    order = d.array("order")
+    nb_g, b_g = r.schedule(order)
+    nb_o, b_o = o.schedule(order)
+    assert nb_g == nb_o
+    assert np.array_equal(b_g, b_o)
+    r.close()
+
+
+def test_single_batch_bitexact(design_c1):
+    d = design_c1
+    r = gpu.Router.from_design(d)
+    o = orc.Oracle.from_design(d)
+    r.set_nets_from_design(d)
+    o.set_nets_from_design(d)
+    dem = _random_demand(r.n, 3, big=False)
+    r.set_demand(dem)
+    o.set_demand(dem)
+    nets = d.array("order")[::7].copy()
+    r.route_batch(nets)
+    o.lshape_batch(nets)
+    assert np.array_equal(r.demand(), o.demand())
+    assert np.array_equal(r.timestamp(), o.timestamp())
+    for n in nets[:2000]:
+        assert np.array_equal(r.route(int(n)), o.route(int(n))), int(n)
+    r.close()
+
+
+def test_pattern_route_c1_bitexact(design_c1):
+    d = design_c1
+    r = gpu.Router.from_design(d)
+    o = orc.Oracle.from_design(d)
+    r.set_nets_from_design(d)
+    o.set_nets_from_design(d)
+    order = d.array("order")
+    st = r.pattern_route(order)
+    nb = o.pattern_route(order)
+    assert st["batches"] == nb
+    assert np.array_equal(r.demand().view(np.uint32), o.demand().view(np.uint32))
+    goff, gdata = r.routes_flat()
+    ooff, odata = o.routes_flat()
+    for i in range(r.num_nets):
+        g = gdata[goff[i]:goff[i] + gdata[goff[i]]]
+        assert np.array_equal(g, odata[ooff[i]:ooff[i + 1]]), i
+    # rip-up marking and congestion view on the routed state
+    assert np.array_equal(r.ripup_mark(0), o.ripup_mark(0))
+    xg, yg = r.congestion_view()
+    xo, yo = o.congestion_view()
+    assert np.array_equal(xg, xo) and np.array_equal(yg, yo)
+    r.close()
