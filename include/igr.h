@@ -91,9 +91,37 @@ int igr_route_batch(igr_ctx* ctx, const int* nets, int64_t n, int form);
  * schedule, then route the batches in the reference order. */
 int igr_pattern_route(igr_ctx* ctx, const int* order, int64_t n, int form, igr_pass_stats* stats);
 
+/* Detour DAGs of the nets of one rip-up-and-reroute pass, one per listed net
+ * in list order, nodes in topological order (parents first): node i of a net
+ * has parents par[par_off[i] .. par_off[i+1]) (indices local to the net,
+ * par_off global over all nodes) and fills child slot slot[k] of parent
+ * par[k]; child_num[i] is node i's slot count and depth[i] its longest-path
+ * depth.  pts are the scheduler points of each net. */
+typedef struct igr_dag_desc {
+    int64_t n;
+    const int64_t* node_off;
+    const int* nodes;
+    const int64_t* par_off;
+    const int* par;
+    const int* slot;
+    const int* child_num;
+    const int* depth;
+    const int64_t* pt_off;
+    const int* pts;
+} igr_dag_desc;
+
+/* One rip-up-and-reroute pass (Lshape_route_detour::Lshape_route_detour_wrap,
+ * src/Lshape_route_detour.hpp:726-1112): 'nets' in routing priority order
+ * (highest first); schedules with the DAG points, then per batch (last
+ * first) rips up the nets' current routes, updates costs and prefix sums,
+ * routes the DAGs and commits the new routes, which replace the old ones. */
+int igr_dag_pass(igr_ctx* ctx, const int* nets, int64_t n, int mode, int form, const igr_dag_desc* dags,
+                 igr_pass_stats* stats);
+
 /* Routes: r[0] = next free slot (1 = empty), then pairs of cell indices. */
 int igr_get_route(igr_ctx* ctx, int net, int* buf, int cap, int* len);
-/* All routes packed: off[n+1] (host), data with data_cap entries. */
+/* All routes packed: off[n+1] (host) where off[i+1]-off[i] is net i's slot
+ * capacity, data with data_cap >= igr_routes_capacity() entries. */
 int igr_get_routes(igr_ctx* ctx, int64_t* off, int* data, int64_t data_cap);
 int64_t igr_routes_capacity(igr_ctx* ctx);
 

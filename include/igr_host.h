@@ -69,6 +69,15 @@ int igrh_grid(const igrh_design* h, igrh_grid_info* out);
  * vs, order, hpwl, ...): returns the element count and sets *ptr. */
 int64_t igrh_array(const igrh_design* h, const char* name, const void** ptr);
 
+/* Detour DAGs of the listed subnets for rip-up and reroute
+ * (net::generate_detours_reconstruct, src/database_cuda.hpp:1429-2086) from
+ * the X*Y congestion prefix views; results in the dag_* arrays, one DAG per
+ * listed net in list order (dag_node_off, dag_node, dag_par_off (global over
+ * nodes), dag_par (local indices), dag_slot, dag_child_num, dag_depth,
+ * dag_pt_off, dag_pt). */
+int igrh_build_dags(igrh_design* h, const int* nets, int64_t n, const float* xsum, const float* ysum, int mode,
+                    int threads);
+
 /* Routes per subnet: off[s] indexes data; data[off[s]] = next free slot,
  * then cell index pairs (routing layers). */
 int igrh_write_routes(const igrh_design* h, const char* out_path, const int64_t* off, const int* data,
@@ -89,5 +98,7 @@ const igr::NetPins* igrh_netpins_ref(const igrh_design* h);
 const igr::Grid* igrh_grid_ref(const igrh_design* h);
 const igr::Subnets* igrh_subnets_ref(const igrh_design* h);
 const igr::PatternInputs* igrh_inputs_ref(const igrh_design* h);
+namespace igr { struct DagSet; }
+const igr::DagSet* igrh_dags_ref(const igrh_design* h);
 #endif
 #endif

@@ -127,11 +127,11 @@ __global__ void __launch_bounds__(TPB) k_commit_edges(GridDev g, int stamp) {
 
 // Via demand of routed nets, gated by the commit timestamps
 // (src/graph.hpp:496-543).
-__global__ void __launch_bounds__(TPB) k_via_commit(GridDev g, const int* nets, int64_t n, const int64_t* route_off,
-                                                    const int* routes, int stamp, int K) {
+__global__ void __launch_bounds__(TPB) k_via_commit(GridDev g, const int* nets, int64_t n, int* const* rptr, int stamp,
+                                                    int K) {
     int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (t >= n) return;
-    const int* r = routes + route_off[nets[t]];
+    const int* r = rptr[nets[t]];
     const int XY = g.X * g.Y;
     const float k = (float)K, h = 0.5f * (float)K;
     for (int i = 1; i < r[0]; i += 2) {
@@ -166,11 +166,11 @@ __global__ void __launch_bounds__(TPB) k_mark_edges(GridDev g, int th, int* of) 
 
 // A net is ripped when more than 10 full edges lie under its wires
 // (src/graph.hpp:441-466).
-__global__ void __launch_bounds__(TPB) k_mark_nets(GridDev g, int64_t n, const int64_t* route_off, const int* routes,
-                                                   const int* of, uint8_t* is_of) {
+__global__ void __launch_bounds__(TPB) k_mark_nets(GridDev g, int64_t n, int* const* rptr, const int* of,
+                                                   uint8_t* is_of) {
     int64_t net = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (net >= n) return;
-    const int* r = routes + route_off[net];
+    const int* r = rptr[net];
     const int XY = g.X * g.Y;
     int cnt = 0;
     uint8_t res = 0;
@@ -290,7 +290,36 @@ void launch_wire_commit(igr_ctx* c, int stamp) {
 
 void launch_via_commit(igr_ctx* c, const int* d_nets, int64_t n, int stamp, int K) {
     if (n == 0) return;
-    k_via_commit<<<blocks(n), TPB, 0, c->stream>>>(c->g, d_nets, n, c->nets.route_off.p, c->nets.routes.p, stamp, K);
+    k_via_commit<<<blocks(n), TPB, 0, c->stream>>>(c->g, d_nets, n, c->nets.rptr.p, stamp, K);
+    HIP_OK(hipGetLastError());
+}
+
+// Rip-up of the wires of routed nets: demand -= 1 on every wire edge and the
+// timestamp set on every wire cell (src/graph.hpp:468-495 with K = -1).
+__global__ void __launch_bounds__(TPB) k_rip_wire(GridDev g, const int* nets, int64_t n, int* const* rptr, int stamp) {
+    int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= n) return;
+    const int* r = rptr[nets[t]];
+    const int XY = g.X * g.Y;
+    for (int i = 1; i < r[0]; i += 2) {
+        int a = r[i], b = r[i + 1];
+        if (a / XY != b / XY) continue;
+        int l = a / XY;
+        int x0 = a / g.Y % g.X, y0 = a % g.Y, x1 = b / g.Y % g.X, y1 = b % g.Y;
+        if (x0 < x1) for (int x = x0; x < x1; x++) { atomicAdd(&g.demand[gidx(g, l, x, y0)], -1.0f); g.ts[gidx(g, l, x, y0)] = stamp; }
+        if (x1 < x0) for (int x = x1; x < x0; x++) { atomicAdd(&g.demand[gidx(g, l, x, y0)], -1.0f); g.ts[gidx(g, l, x, y0)] = stamp; }
+        if (y0 < y1) for (int y = y0; y < y1; y++) { atomicAdd(&g.demand[gidx(g, l, x0, y)], -1.0f); g.ts[gidx(g, l, x0, y)] = stamp; }
+        if (y1 < y0) for (int y = y1; y < y0; y++) { atomicAdd(&g.demand[gidx(g, l, x0, y)], -1.0f); g.ts[gidx(g, l, x0, y)] = stamp; }
+        g.ts[gidx(g, l, x0, y0)] = stamp;
+        g.ts[gidx(g, l, x0, y1)] = stamp;
+        g.ts[gidx(g, l, x1, y0)] = stamp;
+        g.ts[gidx(g, l, x1, y1)] = stamp;
+    }
+}
+
+void launch_rip_wire(igr_ctx* c, const int* d_nets, int64_t n, int stamp) {
+    if (n == 0) return;
+    k_rip_wire<<<blocks(n), TPB, 0, c->stream>>>(c->g, d_nets, n, c->nets.rptr.p, stamp);
     HIP_OK(hipGetLastError());
 }
 
@@ -313,8 +342,7 @@ void launch_ripup_mark(igr_ctx* c, int th, uint8_t* d_is_of, int* d_of) {
     k_mark_edges<<<blocks(n), TPB, 0, c->stream>>>(c->g, th, d_of);
     HIP_OK(hipGetLastError());
     DISPATCH_K(std::max(c->g.X, c->g.Y), k_scan_int, tracks(c->g), c->stream, c->g, d_of);
-    k_mark_nets<<<blocks(c->nets.n), TPB, 0, c->stream>>>(c->g, c->nets.n, c->nets.route_off.p, c->nets.routes.p,
-                                                            d_of, d_is_of);
+    k_mark_nets<<<blocks(c->nets.n), TPB, 0, c->stream>>>(c->g, c->nets.n, c->nets.rptr.p, d_of, d_is_of);
     HIP_OK(hipGetLastError());
 }
 

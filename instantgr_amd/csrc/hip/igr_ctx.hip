@@ -169,6 +169,14 @@ int igr_set_nets(igr_ctx* c, const igr_net_desc* d) {
         std::vector<int> init(N.h_route_off[n], 0);
         for (int64_t i = 0; i < n; i++) init[N.h_route_off[i]] = 1;
         N.routes.upload(init.data(), init.size(), s);
+        N.clear_dag_routes();
+        N.h_rptr.resize(n);
+        N.h_rcap.resize(n);
+        for (int64_t i = 0; i < n; i++) {
+            N.h_rptr[i] = N.routes.p + N.h_route_off[i];
+            N.h_rcap[i] = N.h_route_off[i + 1] - N.h_route_off[i];
+        }
+        N.rptr.upload(N.h_rptr.data(), n, s);
         std::vector<int64_t> z(n + 1, 0);
         N.pt_off.upload(d->pt_off ? d->pt_off : z.data(), n + 1, s);
         N.hs_off.upload(d->hs_off ? d->hs_off : z.data(), n + 1, s);
@@ -231,12 +239,38 @@ void check_nets(const igr_ctx* c, const int* nets, int64_t n) {
         if (nets[i] < 0 || nets[i] >= c->nets.n) throw std::invalid_argument("net id out of range");
 }
 
+// Pattern routing writes the pattern slot: point nets that a rip-up pass
+// moved elsewhere back at it.
+void reset_pattern_ptrs(igr_ctx* c, const int* nets, int64_t n) {
+    NetsDev& N = c->nets;
+    bool dirty = false;
+    for (int64_t i = 0; i < n; i++) {
+        int* p = N.routes.p + N.h_route_off[nets[i]];
+        if (N.h_rptr[nets[i]] != p) {
+            N.h_rptr[nets[i]] = p;
+            N.h_rcap[nets[i]] = N.h_route_off[nets[i] + 1] - N.h_route_off[nets[i]];
+            dirty = true;
+        }
+    }
+    if (dirty) N.rptr.upload(N.h_rptr.data(), N.n, c->stream);
+}
+
+__global__ void k_gather_routes(int* const* rptr, const int64_t* out_off, int* out, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int* r = rptr[i];
+    int* o = out + out_off[i];
+    const int used = r[0];
+    for (int k = 0; k < used; k++) o[k] = r[k];
+}
+
 } // namespace
 
 int igr_route_batch(igr_ctx* c, const int* nets, int64_t n, int form) {
     return guard([&]() {
         check_nets(c, nets, n);
         if (form != IGR_FORM_PLAIN && form != IGR_FORM_BATCHED) throw std::invalid_argument("unknown form");
+        reset_pattern_ptrs(c, nets, n);
         std::vector<int64_t> off(n);
         int64_t tot = scratch_offsets(c, nets, n, off.data());
         ensure_scratch(c, tot);
@@ -251,6 +285,7 @@ int igr_pattern_route(igr_ctx* c, const int* order, int64_t n, int form, igr_pas
     return guard([&]() {
         check_nets(c, order, n);
         if (form != IGR_FORM_PLAIN && form != IGR_FORM_BATCHED) throw std::invalid_argument("unknown form");
+        reset_pattern_ptrs(c, order, n);
         double t0 = now_s();
         std::vector<int> batch_of(n);
         int nb = run_schedule(c, order, n, 0, batch_of.data());
@@ -287,10 +322,9 @@ int igr_pattern_route(igr_ctx* c, const int* order, int64_t n, int form, igr_pas
 int igr_get_route(igr_ctx* c, int net, int* buf, int cap, int* len) {
     return guard([&]() {
         if (net < 0 || net >= c->nets.n) throw std::invalid_argument("net id out of range");
-        int64_t b = c->nets.h_route_off[net], e = c->nets.h_route_off[net + 1];
-        std::vector<int> tmp(e - b);
-        HIP_OK(hipMemcpyAsync(tmp.data(), c->nets.routes.p + b, sizeof(int) * (e - b), hipMemcpyDeviceToHost,
-                              c->stream));
+        const int64_t rc = c->nets.h_rcap[net];
+        std::vector<int> tmp(rc);
+        HIP_OK(hipMemcpyAsync(tmp.data(), c->nets.h_rptr[net], sizeof(int) * rc, hipMemcpyDeviceToHost, c->stream));
         HIP_OK(hipStreamSynchronize(c->stream));
         int used = tmp[0];
         if (len) *len = used;
@@ -298,15 +332,183 @@ int igr_get_route(igr_ctx* c, int net, int* buf, int cap, int* len) {
     });
 }
 
-int64_t igr_routes_capacity(igr_ctx* c) { return c->nets.h_route_off.empty() ? 0 : c->nets.h_route_off.back(); }
+int64_t igr_routes_capacity(igr_ctx* c) {
+    int64_t t = 0;
+    for (int64_t v : c->nets.h_rcap) t += v;
+    return t;
+}
 
 int igr_get_routes(igr_ctx* c, int64_t* off, int* data, int64_t cap) {
     return guard([&]() {
-        const int64_t n = c->nets.n, tot = c->nets.h_route_off[n];
-        if (cap < tot) throw std::invalid_argument("route buffer too small");
-        HIP_OK(hipMemcpyAsync(data, c->nets.routes.p, sizeof(int) * tot, hipMemcpyDeviceToHost, c->stream));
+        const int64_t n = c->nets.n;
+        std::vector<int64_t> o(n + 1, 0);
+        for (int64_t i = 0; i < n; i++) o[i + 1] = o[i] + c->nets.h_rcap[i];
+        if (cap < o[n]) throw std::invalid_argument("route buffer too small");
+        DevBuf<int64_t> doff;
+        DevBuf<int> out;
+        doff.upload(o.data(), n + 1, c->stream);
+        out.alloc(o[n]);
+        if (n) k_gather_routes<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(c->nets.rptr.p, doff.p, out.p, n);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(data, out.p, sizeof(int) * o[n], hipMemcpyDeviceToHost, c->stream));
         HIP_OK(hipStreamSynchronize(c->stream));
-        std::memcpy(off, c->nets.h_route_off.data(), sizeof(int64_t) * (n + 1));
+        std::memcpy(off, o.data(), sizeof(int64_t) * (n + 1));
+    });
+}
+
+int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, const igr_dag_desc* dg,
+                 igr_pass_stats* st) {
+    return guard([&]() {
+        check_nets(c, nets, n);
+        if (form != IGR_FORM_PLAIN && form != IGR_FORM_BATCHED) throw std::invalid_argument("unknown form");
+        if (!dg || dg->n != n) throw std::invalid_argument("one DAG per listed net");
+        if (n == 0) {
+            if (st) *st = igr_pass_stats{0, 0, 0, 0};
+            return;
+        }
+        const int XY = c->g.X * c->g.Y, L = c->g.L, LL = L * L, ML = L + 1;
+        // Validate the DAGs before anything reaches the device.
+        for (int64_t i = 0; i < n; i++) {
+            const int64_t b = dg->node_off[i], e = dg->node_off[i + 1];
+            if (e <= b) throw std::invalid_argument("empty DAG");
+            for (int64_t k = b; k < e; k++) {
+                if (dg->nodes[k] < 0 || dg->nodes[k] >= (L + 1) * XY) throw std::invalid_argument("DAG node outside the grid");
+                if (dg->child_num[k] < 0 || dg->child_num[k] > 10) throw std::invalid_argument("DAG node with > 10 slots");
+                if ((k == b) != (dg->par_off[k + 1] == dg->par_off[k]))
+                    throw std::invalid_argument("DAG node 0 must be the only source");
+                for (int64_t q = dg->par_off[k]; q < dg->par_off[k + 1]; q++) {
+                    int p = dg->par[q];
+                    if (p < 0 || p >= k - b) throw std::invalid_argument("DAG parents must precede children");
+                    if (dg->slot[q] < 0 || dg->slot[q] >= dg->child_num[b + p]) throw std::invalid_argument("bad DAG slot");
+                }
+            }
+            for (int64_t q = dg->pt_off[i]; q < dg->pt_off[i + 1]; q++)
+                if (dg->pts[q] < 0 || dg->pts[q] >= XY) throw std::invalid_argument("DAG point outside the grid");
+        }
+        DagDev& D = c->dag;
+        hipStream_t s = c->stream;
+        double t0 = now_s();
+        D.pt_off.upload(dg->pt_off, n + 1, s);
+        D.pt.upload(dg->pts, dg->pt_off[n], s);
+        std::vector<int> batch_of(n);
+        int nb = run_schedule(c, nets, n, mode, batch_of.data(), D.pt_off.p, D.pt.p);
+        double t1 = now_s();
+
+        // Routing order: batch nb-1 first, inside a batch decreasing list
+        // position (the scheduler's index order).
+        std::vector<std::vector<int64_t>> bl(nb);
+        for (int64_t i = n - 1; i >= 0; i--) bl[batch_of[i]].push_back(i);
+        std::vector<int> rnets;
+        std::vector<int64_t> node_off(1, 0), par_off(1, 0), csum(1, 0), route_slot;
+        std::vector<int> node, par, slot, child_num, loc, enode, lvl_nodes;
+        std::vector<int64_t> lvl_off(1, 0);
+        std::vector<DagBatch> batches;
+        int64_t route_tot = 0;
+        int64_t mx_nodes = 0, mx_edges = 0, mx_slots = 0;
+        for (int b = nb - 1; b >= 0; b--) {
+            DagBatch B{};
+            B.net0 = (int64_t)rnets.size();
+            B.node0 = (int64_t)node.size();
+            B.edge0 = (int64_t)par.size();
+            B.slot0 = csum.back();
+            int maxd = 0;
+            for (int64_t i : bl[b]) {
+                const int64_t nb0 = dg->node_off[i], nb1 = dg->node_off[i + 1];
+                const int64_t g0 = (int64_t)node.size();
+                rnets.push_back(nets[i]);
+                route_slot.push_back(route_tot);
+                route_tot += 8 * (nb1 - nb0) + 1;
+                for (int64_t k = nb0; k < nb1; k++) {
+                    node.push_back(dg->nodes[k]);
+                    child_num.push_back(dg->child_num[k]);
+                    loc.push_back((int)(k - nb0));
+                    csum.push_back(csum.back() + dg->child_num[k]);
+                    maxd = std::max(maxd, dg->depth[k]);
+                    for (int64_t q = dg->par_off[k]; q < dg->par_off[k + 1]; q++) {
+                        par.push_back((int)(g0 + dg->par[q]));
+                        slot.push_back(dg->slot[q]);
+                        enode.push_back((int)(g0 + (k - nb0)));
+                    }
+                    par_off.push_back((int64_t)par.size());
+                }
+                node_off.push_back((int64_t)node.size());
+            }
+            B.net1 = (int64_t)rnets.size();
+            B.node1 = (int64_t)node.size();
+            B.edge1 = (int64_t)par.size();
+            B.slot1 = csum.back();
+            B.levels = maxd + 1;
+            B.lvl0 = (int64_t)lvl_off.size() - 1;
+            // Nodes of the batch bucketed by depth (stable in node order).
+            std::vector<int64_t> cnt(B.levels + 1, 0);
+            std::vector<int> dep(B.node1 - B.node0);
+            {
+                int64_t k = 0;
+                for (int64_t i : bl[b])
+                    for (int64_t q = dg->node_off[i]; q < dg->node_off[i + 1]; q++) dep[k++] = dg->depth[q];
+            }
+            for (int dd : dep) cnt[dd + 1]++;
+            for (int d = 0; d < B.levels; d++) cnt[d + 1] += cnt[d];
+            std::vector<int> bucket(dep.size());
+            {
+                std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+                for (size_t k = 0; k < dep.size(); k++) bucket[pos[dep[k]]++] = (int)(B.node0 + k);
+            }
+            const int64_t base = (int64_t)lvl_nodes.size();
+            lvl_nodes.insert(lvl_nodes.end(), bucket.begin(), bucket.end());
+            for (int d = 0; d < B.levels; d++) lvl_off.push_back(base + cnt[d + 1]);
+            mx_nodes = std::max(mx_nodes, B.node1 - B.node0);
+            mx_edges = std::max(mx_edges, B.edge1 - B.edge0);
+            mx_slots = std::max(mx_slots, B.slot1 - B.slot0);
+            batches.push_back(B);
+        }
+        // lvl_off holds, per batch, levels+1 boundaries sharing endpoints.
+        D.nets.upload(rnets.data(), rnets.size(), s);
+        D.node.upload(node.data(), node.size(), s);
+        D.par.upload(par.data(), par.size(), s);
+        D.slot.upload(slot.data(), slot.size(), s);
+        D.child_num.upload(child_num.data(), child_num.size(), s);
+        D.loc.upload(loc.data(), loc.size(), s);
+        D.enode.upload(enode.data(), enode.size(), s);
+        D.lvl_nodes.upload(lvl_nodes.data(), lvl_nodes.size(), s);
+        D.node_off.upload(node_off.data(), node_off.size(), s);
+        D.par_off.upload(par_off.data(), par_off.size(), s);
+        D.csum.upload(csum.data(), csum.size(), s);
+        D.route_slot.upload(route_slot.data(), route_slot.size(), s);
+        D.costs.alloc(mx_nodes * ML);
+        D.lo.alloc(mx_nodes);
+        D.emit.alloc(mx_nodes * 9);
+        D.packed.alloc(std::max<int64_t>(1, mx_slots) * ML);
+        D.best.alloc(std::max<int64_t>(1, mx_slots) * ML);
+        D.cost_edges.alloc(std::max<int64_t>(1, mx_edges) * LL);
+        D.best_change.alloc(std::max<int64_t>(1, mx_edges) * LL);
+        NetsDev& N = c->nets;
+        N.dag_routes.push_back(new DevBuf<int>());
+        N.dag_routes.back()->alloc(route_tot);
+        int* rbase = N.dag_routes.back()->p;
+
+        for (const DagBatch& B : batches) {
+            const int* bn = D.nets.p + B.net0;
+            const int64_t cnt = B.net1 - B.net0;
+            const int s1 = ++c->stamp;
+            launch_rip_wire(c, bn, cnt, s1);
+            launch_via_commit(c, bn, cnt, s1, -1);
+            const int s2 = ++c->stamp;
+            launch_update_cost(c);
+            launch_presum(c);
+            launch_dag_batch_plain(c, B, s2, lvl_off);
+            launch_dag_compact(c, B);
+            launch_wire_commit(c, s2);
+            launch_via_commit(c, bn, cnt, s2, 1);
+            if (form == IGR_FORM_PLAIN) HIP_OK(hipStreamSynchronize(s));
+        }
+        HIP_OK(hipStreamSynchronize(s));
+        for (size_t k = 0; k < rnets.size(); k++) {
+            N.h_rptr[rnets[k]] = rbase + route_slot[k];
+            N.h_rcap[rnets[k]] = (k + 1 < route_slot.size() ? route_slot[k + 1] : route_tot) - route_slot[k];
+        }
+        double t2 = now_s();
+        if (st) *st = igr_pass_stats{nb, t1 - t0, t2 - t1, n};
     });
 }
 

@@ -49,18 +49,43 @@ struct DevBuf {
     }
 };
 
-// Net trees and footprints on the device (pattern pass).
+// Net trees and footprints on the device (pattern pass).  Every net's current
+// route is reached through rptr[net]: pattern routes live in 'routes' at
+// route_off[net]; a rip-up-and-reroute pass moves the nets it reroutes to a
+// buffer of its own (dag_routes).
 struct NetsDev {
     int64_t n = 0;
     DevBuf<int64_t> node_off, route_off, pt_off, hs_off, vs_off;
     DevBuf<int> node, parent, routes, pt, hs, vs;
+    DevBuf<int*> rptr;
     std::vector<int64_t> h_node_off, h_route_off;
     std::vector<int> h_node;          // host copies used by the host-side batch logic
+    std::vector<int*> h_rptr;
+    std::vector<int64_t> h_rcap;      // capacity of each net's current route slot
+    std::vector<DevBuf<int>*> dag_routes;
     int max_nodes = 0;
+    ~NetsDev() { clear_dag_routes(); }
+    void clear_dag_routes() {
+        for (auto* b : dag_routes) delete b;
+        dag_routes.clear();
+    }
 };
 
 struct Sched {
-    DevBuf<int> map, batch, status, viable, ids, flags;
+    DevBuf<int> map, batch, status, viable, ids, pids, flags;
+};
+
+// One rip-up-and-reroute pass on the device: DAGs of the rerouted nets in
+// routing order (batch by batch), with global node/edge/slot numbering.
+struct DagDev {
+    DevBuf<int> node, par, slot, child_num, loc, enode, node_net, lvl_nodes, nets;
+    DevBuf<int64_t> node_off, par_off, csum, route_slot;
+    DevBuf<int64_t> lvl_off;
+    DevBuf<double> costs, cost_edges;
+    DevBuf<uint64_t> packed, best;
+    DevBuf<int> lo, best_change, emit;
+    DevBuf<int> pt;
+    DevBuf<int64_t> pt_off;
 };
 
 struct Scratch {
@@ -87,6 +112,7 @@ struct igr_ctx {
     igr_dev::NetsDev nets;
     igr_dev::Sched sched;
     igr_dev::Scratch scratch;
+    igr_dev::DagDev dag;
     int stamp = 0;
     int64_t cells() const { return (int64_t)g.L * g.X * g.Y; }
 };
@@ -102,6 +128,19 @@ void launch_to_ref(igr_ctx* c, const void* src, void* dst, int elem_bytes);
 void launch_from_ref(igr_ctx* c, const void* src, void* dst, int elem_bytes);
 void launch_ripup_mark(igr_ctx* c, int th, uint8_t* d_is_of, int* d_of);
 void launch_congestion(igr_ctx* c, float* d_xsum, float* d_ysum);
-int run_schedule(igr_ctx* c, const int* h_order, int64_t n, int mode, int* h_batch_of);
+// Scheduler; with d_pt_off/d_pt given, the points of order[i] are list entry i
+// of those arrays instead of the pattern footprint.
+int run_schedule(igr_ctx* c, const int* h_order, int64_t n, int mode, int* h_batch_of,
+                 const int64_t* d_pt_off = nullptr, const int* d_pt = nullptr);
+void launch_rip_wire(igr_ctx* c, const int* d_nets, int64_t n, int stamp);
+// Plain (reference launch structure) device work of one DAG batch, after the
+// rip-up and the cost/prefix update.
+struct DagBatch {
+    int64_t net0, net1, node0, node1, edge0, edge1, slot0, slot1;
+    int levels;
+    int64_t lvl0;               // first entry of this batch in lvl_off
+};
+void launch_dag_batch_plain(igr_ctx* c, const DagBatch& b, int stamp, const std::vector<int64_t>& h_lvl_off);
+void launch_dag_compact(igr_ctx* c, const DagBatch& b);
 void launch_expf(const float* in, float* out, int64_t n, hipStream_t s);
 } // namespace igr_dev

@@ -25,7 +25,8 @@ constexpr int TPB = 256;
 
 struct SchedArgs {
     int64_t n;
-    const int* ids;              // scheduler index -> net id
+    const int* ids;              // scheduler index -> net id (segments)
+    const int* pids;             // scheduler index -> row of the point arrays
     const int64_t* pt_off;
     const int* pt;
     const int64_t* hs_off;
@@ -46,8 +47,7 @@ __global__ void __launch_bounds__(TPB) k_claim_points(SchedArgs a) {
     int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (i >= a.n || !active(a, i)) return;
     a.status[i] = 1;
-    const int net = a.ids[i];
-    for (int64_t k = a.pt_off[net]; k < a.pt_off[net + 1]; k++) atomicMax(&a.map[a.pt[k]], (int)i);
+    for (int64_t k = a.pt_off[a.pids[i]]; k < a.pt_off[a.pids[i] + 1]; k++) atomicMax(&a.map[a.pt[k]], (int)i);
 }
 
 __global__ void __launch_bounds__(TPB) k_viable(SchedArgs a) {
@@ -55,9 +55,8 @@ __global__ void __launch_bounds__(TPB) k_viable(SchedArgs a) {
     if (i >= a.n) return;
     int ok = 0;
     if (active(a, i)) {
-        const int net = a.ids[i];
         ok = 1;
-        for (int64_t k = a.pt_off[net]; k < a.pt_off[net + 1]; k++)
+        for (int64_t k = a.pt_off[a.pids[i]]; k < a.pt_off[a.pids[i] + 1]; k++)
             if (a.map[a.pt[k]] != (int)i) { ok = 0; break; }
     }
     a.viable[i] = ok;
@@ -68,7 +67,7 @@ __global__ void __launch_bounds__(TPB) k_claim_rest(SchedArgs a) {
     if (i >= a.n || !a.viable[i]) return;
     const int net = a.ids[i];
     const int X = a.X, Y = a.Y;
-    for (int64_t k = a.pt_off[net]; k < a.pt_off[net + 1]; k++) {
+    for (int64_t k = a.pt_off[a.pids[i]]; k < a.pt_off[a.pids[i] + 1]; k++) {
         int p = a.pt[k], x = p / Y, y = p % Y;
         if (y - 1 >= 0) atomicMax(&a.map[x * Y + y - 1], (int)i);
         if (x - 1 >= 0) atomicMax(&a.map[(x - 1) * Y + y], (int)i);
@@ -87,8 +86,7 @@ __global__ void __launch_bounds__(TPB) k_claim_rest(SchedArgs a) {
 __global__ void __launch_bounds__(TPB) k_commit(SchedArgs a, int cur) {
     int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (i >= a.n || !active(a, i)) return;
-    const int net = a.ids[i];
-    for (int64_t k = a.pt_off[net]; k < a.pt_off[net + 1]; k++)
+    for (int64_t k = a.pt_off[a.pids[i]]; k < a.pt_off[a.pids[i] + 1]; k++)
         if (a.map[a.pt[k]] != (int)i) return;
     a.batch[i] = cur;
     a.flags[0] = 1;
@@ -97,8 +95,7 @@ __global__ void __launch_bounds__(TPB) k_commit(SchedArgs a, int cur) {
 __global__ void __launch_bounds__(TPB) k_revert(SchedArgs a) {
     int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (i >= a.n || a.status[i] != 1 || a.batch[i] != -1) return;
-    const int net = a.ids[i];
-    for (int64_t k = a.pt_off[net]; k < a.pt_off[net + 1]; k++) {
+    for (int64_t k = a.pt_off[a.pids[i]]; k < a.pt_off[a.pids[i] + 1]; k++) {
         int pre = a.map[a.pt[k]];
         if (pre != -1 && a.batch[pre] != -1) { a.status[i] = 2; return; }
     }
@@ -120,7 +117,8 @@ inline unsigned blocks(int64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
 
 } // namespace
 
-int run_schedule(igr_ctx* c, const int* h_order, int64_t n, int mode, int* h_batch_of) {
+int run_schedule(igr_ctx* c, const int* h_order, int64_t n, int mode, int* h_batch_of, const int64_t* d_pt_off,
+                 const int* d_pt) {
     if (n == 0) return 0;
     Sched& S = c->sched;
     NetsDev& N = c->nets;
@@ -128,13 +126,19 @@ int run_schedule(igr_ctx* c, const int* h_order, int64_t n, int mode, int* h_bat
     std::vector<int> ids(n);
     for (int64_t i = 0; i < n; i++) ids[i] = h_order[n - 1 - i];
     S.ids.upload(ids.data(), n, st);
+    std::vector<int> pids;
+    if (d_pt_off) {
+        pids.resize(n);
+        for (int64_t i = 0; i < n; i++) pids[i] = (int)(n - 1 - i);
+        S.pids.upload(pids.data(), n, st);
+    }
     const int64_t XYc = (int64_t)c->g.X * c->g.Y;
     S.map.alloc(XYc);
     S.batch.alloc(n);
     S.status.alloc(n);
     S.viable.alloc(n);
     S.flags.alloc(4);
-    SchedArgs a{n, S.ids.p, N.pt_off.p, N.pt.p, N.hs_off.p, N.hs.p, N.vs_off.p, N.vs.p,
+    SchedArgs a{n, S.ids.p, d_pt_off ? S.pids.p : S.ids.p, d_pt_off ? d_pt_off : N.pt_off.p, d_pt_off ? d_pt : N.pt.p, N.hs_off.p, N.hs.p, N.vs_off.p, N.vs.p,
                 S.map.p, S.batch.p, S.status.p, S.viable.p, S.flags.p, c->g.X, c->g.Y,
                 (mode == 1 || mode == 2) ? 2 : 15};
     k_fill<<<blocks(n), TPB, 0, st>>>(S.batch.p, n, -1);

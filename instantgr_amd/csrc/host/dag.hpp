@@ -1,0 +1,30 @@
+// Detour DAGs for rip-up and reroute (host construction).
+#pragma once
+#include "design.hpp"
+
+namespace igr {
+
+// One net's DAG in topological order (parents before children): node i has
+// parents par[par_off[i] .. par_off[i+1]) (local indices) and occupies child
+// slot slot[k] of parent par[k]; child_num[i] is its number of child slots,
+// depth[i] its longest-path depth from the source (node 0).  Nodes are
+// l*X*Y + x*Y + y with l = L for nodes that are not pins.
+struct DagNet {
+    std::vector<int> node, par_off, par, slot, child_num, depth, points;
+};
+
+// DAGs of several nets concatenated; par_off is global over all nodes and
+// par holds local (per-net) node indices.
+struct DagSet {
+    int64_t n = 0;
+    std::vector<int64_t> node_off, par_off, pt_off;
+    std::vector<int> node, par, slot, child_num, depth, pt;
+};
+
+void build_dag(const Grid& g, const float* xsum, const float* ysum, int mode, const int* tnode, int tn,
+               const int64_t* adj_off, const int* tadj, DagNet& out);
+
+void build_dags(const Grid& g, const Subnets& s, const int* nets, int64_t n, const float* xsum, const float* ysum,
+                int mode, int threads, DagSet& out);
+
+} // namespace igr

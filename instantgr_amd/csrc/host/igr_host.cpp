@@ -3,10 +3,12 @@
 // evaluator.  Used by the C++ driver, the GPU library and the Python tests.
 #include "igr_host.h"
 #include "prep.hpp"
+#include "dag.hpp"
 
 #include <chrono>
 #include <cstring>
 #include <exception>
+#include <stdexcept>
 #include <string>
 
 using namespace igr;
@@ -17,6 +19,7 @@ struct igrh_design {
     Grid g;
     Subnets s;
     PatternInputs pi;
+    DagSet dags;
     bool built = false;
     double t_tree = 0;
 };
@@ -118,10 +121,23 @@ int64_t igrh_array(const igrh_design* h, const char* name, const void** ptr) {
     A("net_sub_off", s.net_sub_off) A("net_sub", s.net_sub)
     A("pin_off", h->np.pin_off) A("pins", h->np.pins) A("extra_off", h->np.extra_off) A("extra", h->np.extra)
     A("ap_off", h->d.ap_off) A("ap", h->d.ap) A("net_pin_off", h->d.pin_off)
+    A("dag_node_off", h->dags.node_off) A("dag_node", h->dags.node) A("dag_par_off", h->dags.par_off)
+    A("dag_par", h->dags.par) A("dag_slot", h->dags.slot) A("dag_child_num", h->dags.child_num)
+    A("dag_depth", h->dags.depth) A("dag_pt_off", h->dags.pt_off) A("dag_pt", h->dags.pt)
 #undef A
     g_err = "unknown array " + n;
     *ptr = nullptr;
     return -1;
+}
+
+int igrh_build_dags(igrh_design* h, const int* nets, int64_t n, const float* xsum, const float* ysum, int mode,
+                     int threads) {
+    return guard([&]() {
+        if (!h->built) throw std::runtime_error("design not built");
+        for (int64_t i = 0; i < n; i++)
+            if (nets[i] < 0 || nets[i] >= h->s.count) throw std::runtime_error("net id out of range");
+        build_dags(h->g, h->s, nets, n, xsum, ysum, mode, threads, h->dags);
+    });
 }
 
 int igrh_write_routes(const igrh_design* h, const char* out_path, const int64_t* off, const int* data,
@@ -148,3 +164,4 @@ const igr::NetPins* igrh_netpins_ref(const igrh_design* h) { return &h->np; }
 const igr::Grid* igrh_grid_ref(const igrh_design* h) { return &h->g; }
 const igr::Subnets* igrh_subnets_ref(const igrh_design* h) { return &h->s; }
 const igr::PatternInputs* igrh_inputs_ref(const igrh_design* h) { return &h->pi; }
+const igr::DagSet* igrh_dags_ref(const igrh_design* h) { return &h->dags; }

@@ -25,6 +25,12 @@ class NetDesc(C.Structure):
                                       ("node_off", "nodes", "parents", "pt_off", "pts", "hs_off", "hs", "vs_off", "vs")]
 
 
+class DagDesc(C.Structure):
+    _fields_ = [("n", C.c_int64)] + [(k, C.c_void_p) for k in
+                                      ("node_off", "nodes", "par_off", "par", "slot", "child_num", "depth",
+                                       "pt_off", "pts")]
+
+
 class PassStats(C.Structure):
     _fields_ = [("batches", C.c_int), ("schedule_s", C.c_double), ("route_s", C.c_double), ("nets", C.c_int64)]
 
@@ -44,6 +50,8 @@ SIGNATURES = {
     "igr_schedule": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.POINTER(C.c_int)]),
     "igr_route_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int]),
     "igr_pattern_route": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.POINTER(PassStats)]),
+    "igr_dag_pass": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.POINTER(DagDesc),
+                               C.POINTER(PassStats)]),
     "igr_get_route": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "igr_get_routes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
     "igr_routes_capacity": (C.c_int64, [C.c_void_p]),
@@ -177,6 +185,18 @@ class Router:
         _check(lib().igr_pattern_route(self.h, order.ctypes.data, len(order), form, C.byref(st)))
         return {"batches": st.batches, "schedule_s": st.schedule_s, "route_s": st.route_s, "nets": st.nets}
 
+    def dag_pass(self, nets, dags, mode=0, form=FORM_PLAIN):
+        """One rip-up-and-reroute pass over the DAGs returned by
+        instantgr_amd.host.Design.build_dags for the same net list."""
+        nets = np.ascontiguousarray(nets, np.int32)
+        keys = ("dag_node_off", "dag_node", "dag_par_off", "dag_par", "dag_slot", "dag_child_num", "dag_depth",
+                "dag_pt_off", "dag_pt")
+        arrs = [np.ascontiguousarray(dags[k]) for k in keys]
+        d = DagDesc(len(nets), *[a.ctypes.data for a in arrs])
+        st = PassStats()
+        _check(lib().igr_dag_pass(self.h, nets.ctypes.data, len(nets), mode, form, C.byref(d), C.byref(st)))
+        return {"batches": st.batches, "schedule_s": st.schedule_s, "route_s": st.route_s, "nets": st.nets}
+
     def route(self, net):
         buf = np.empty(65536, np.int32)
         ln = C.c_int()
@@ -184,6 +204,7 @@ class Router:
         return buf[:ln.value].copy()
 
     def routes_flat(self):
+        """(off, data): net i's route is data[off[i] : off[i] + data[off[i]]]."""
         cap = lib().igr_routes_capacity(self.h)
         off = np.empty(self.num_nets + 1, np.int64)
         data = np.empty(max(cap, 1), np.int32)

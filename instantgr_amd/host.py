@@ -53,6 +53,7 @@ def lib():
         L.igrh_array.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]
         L.igrh_array.restype = C.c_int64
         L.igrh_write_routes.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p, C.c_int64]
+        L.igrh_build_dags.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
         L.igrh_evaluate.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(Score)]
         _lib = L
     return _lib
@@ -65,7 +66,8 @@ def _check(rc):
 
 _DTYPES = {"node_off": np.int64, "pt_off": np.int64, "hs_off": np.int64, "vs_off": np.int64,
            "sub_pin_off": np.int64, "tree_node_off": np.int64, "tree_adj_off": np.int64, "net_sub_off": np.int64,
-           "pin_off": np.int64, "extra_off": np.int64, "ap_off": np.int64, "net_pin_off": np.int64}
+           "pin_off": np.int64, "extra_off": np.int64, "ap_off": np.int64, "net_pin_off": np.int64,
+           "dag_node_off": np.int64, "dag_par_off": np.int64, "dag_pt_off": np.int64}
 
 
 def gen_params(layers=10, xsize=64, ysize=64, nets=500, pin_cap=30, geom_p=0.35, sigma=8.0,
@@ -136,6 +138,17 @@ class Design:
         xlen = np.ctypeslib.as_array(i.xlen, shape=(i.X - 1,))
         ylen = np.ctypeslib.as_array(i.ylen, shape=(i.Y - 1,))
         return cap, w, xlen, ylen
+
+    def build_dags(self, nets, xsum, ysum, mode=0, threads=8):
+        nets = np.ascontiguousarray(nets, np.int32)
+        xsum = np.ascontiguousarray(xsum, np.float32)
+        ysum = np.ascontiguousarray(ysum, np.float32)
+        _check(lib().igrh_build_dags(self.h, nets.ctypes.data, len(nets), xsum.ctypes.data, ysum.ctypes.data,
+                                     mode, threads))
+        for k in [k for k in self._arrays if k.startswith("dag_")]:
+            del self._arrays[k]
+        return {k: self.array(k).copy() for k in ("dag_node_off", "dag_node", "dag_par_off", "dag_par", "dag_slot",
+                                                  "dag_child_num", "dag_depth", "dag_pt_off", "dag_pt")}
 
     def write_routes(self, path, off, data):
         off = np.ascontiguousarray(off, dtype=np.int64)

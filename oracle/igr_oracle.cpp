@@ -437,11 +437,19 @@ void orc_lshape_batch(orc_ctx* c, const int* nets, int64_t n) {
 // their neighbour cells and sampled segment cells; nets that lost a point
 // claim nothing more.  Commit, revert and clear follow the reference.
 // ----------------------------------------------------------------------------
-int orc_schedule(orc_ctx* c, const int* order, int64_t n, int mode, int* batch_of) {
+namespace {
+int schedule_impl(orc_ctx* c, const int* order, int64_t n, int mode, const int64_t* pt_off, const int* pt,
+                  int* batch_of) {
     const int X = c->X, Y = c->Y;
     const int sparsity = (mode == 1 || mode == 2) ? 2 : 15;
     std::vector<int> ids(n);
     for (int64_t i = 0; i < n; i++) ids[i] = order[n - 1 - i];
+    std::vector<std::vector<int>> own;
+    if (pt_off) {
+        own.resize(n);
+        for (int64_t i = 0; i < n; i++) own[i].assign(pt + pt_off[n - 1 - i], pt + pt_off[n - i]);
+    }
+    auto pts_of = [&](int64_t i) -> const std::vector<int>& { return pt_off ? own[i] : c->nets[ids[i]].pts; };
     std::vector<int> batch(n, -1), status(n, 0), map((size_t)X * Y, -1);
     const int xs[4] = {0, 0, -1, 1}, ys[4] = {0, -1, 0, 0};
     int current = 0;
@@ -455,19 +463,19 @@ int orc_schedule(orc_ctx* c, const int* order, int64_t n, int mode, int* batch_o
             for (int64_t i = 0; i < n; i++) {
                 if (!active(i)) continue;
                 status[i] = 1;
-                for (int p : c->nets[ids[i]].pts) map[p] = std::max(map[p], (int)i);
+                for (int p : pts_of(i)) map[p] = std::max(map[p], (int)i);
             }
             std::vector<int64_t> viable;
             for (int64_t i = 0; i < n; i++) {
                 if (!active(i)) continue;
                 bool ok = true;
-                for (int p : c->nets[ids[i]].pts)
+                for (int p : pts_of(i))
                     if (map[p] != (int)i) { ok = false; break; }
                 if (ok) viable.push_back(i);
             }
             for (int64_t i : viable) {
                 const Net& t = c->nets[ids[i]];
-                for (int p : t.pts) {
+                for (int p : pts_of(i)) {
                     int x = p / Y, y = p % Y;
                     for (int j = 1; j < 4; j++) {
                         int xn = x + xs[j], yn = y + ys[j];
@@ -488,14 +496,14 @@ int orc_schedule(orc_ctx* c, const int* order, int64_t n, int mode, int* batch_o
             for (int64_t i = 0; i < n; i++) {
                 if (!active(i)) continue;
                 bool ok = true;
-                for (int p : c->nets[ids[i]].pts)
+                for (int p : pts_of(i))
                     if (map[p] != (int)i) { ok = false; break; }
                 if (ok) { batch[i] = current; have_commit = have_commit2 = true; }
             }
             if (!have_commit2) break;
             for (int64_t i = 0; i < n; i++) {
                 if (status[i] != 1 || batch[i] != -1) continue;
-                for (int p : c->nets[ids[i]].pts) {
+                for (int p : pts_of(i)) {
                     int pre = map[p];
                     if (pre != -1 && batch[pre] != -1) { status[i] = 2; break; }
                 }
@@ -508,6 +516,11 @@ int orc_schedule(orc_ctx* c, const int* order, int64_t n, int mode, int* batch_o
     }
     for (int64_t i = 0; i < n; i++) batch_of[n - 1 - i] = batch[i];
     return current;
+}
+} // namespace
+
+int orc_schedule(orc_ctx* c, const int* order, int64_t n, int mode, int* batch_of) {
+    return schedule_impl(c, order, n, mode, nullptr, nullptr, batch_of);
 }
 
 // Pattern pass (src/Lshape_route.hpp:155-257): schedule, reverse the batch
@@ -586,4 +599,309 @@ void orc_congestion_view(orc_ctx* c, float* xsum, float* ysum) {
         sklansky(s);
         for (int y = 0; y < Y; y++) ysum[(size_t)x * Y + y] = s[y];
     }
+}
+
+// ----------------------------------------------------------------------------
+// Rip-up and reroute over detour DAGs (src/Lshape_route_detour.hpp).
+// ----------------------------------------------------------------------------
+namespace {
+
+// Ordered-int packing of an FP32 cost and a node id into one 64-bit key
+// (src/Lshape_route_detour.hpp:83-104): smaller key = smaller cost, then
+// smaller id.
+inline uint64_t pack_key(float cost, uint32_t id) {
+    int32_t i;
+    std::memcpy(&i, &cost, 4);
+    int32_t o = i >= 0 ? i : i ^ 0x7FFFFFFF;
+    return ((uint64_t)(int64_t)o << 32) | (uint64_t)id;
+}
+inline void unpack_key(uint64_t k, double* cost, uint32_t* id) {
+    int32_t o = (int32_t)(k >> 32);
+    int32_t i = o >= 0 ? o : o ^ 0x7FFFFFFF;
+    float f;
+    std::memcpy(&f, &i, 4);
+    *cost = (double)f;
+    *id = (uint32_t)(k & 0xFFFFFFFFULL);
+}
+// Initial slot key (src/Lshape_route_detour.hpp:106).
+constexpr uint64_t INIT_KEY = 7208259049189261824ULL;
+// Id multiplier and the "straight edge" marker: the reference uses 100 and 82
+// with 9 routing layers; any multiplier above every choice code keeps the
+// (node, choice) order, and the marker must exceed every layer-pair code.
+constexpr uint32_t ID_MUL = 1000;
+
+struct DagView {
+    int n;                   // nodes
+    const int* node;         // encoded positions
+    const int64_t* par_off;  // n+1 (local offsets)
+    const int* par;          // local parent index per edge
+    const int* slot;         // child slot per edge
+    const int* child_num;
+    const int* depth;
+};
+
+// One net's DAG route (src/Lshape_route_detour.hpp:123-195 precompute_cost_edge,
+// :281-406 Lshape_route_node_layer_change_cuda, :520-694
+// get_routing_tree_layer_change_cuda).  Writes the route and the wire
+// difference array.
+void dag_route_net(orc_ctx* c, const DagView& d, std::vector<int>& r, int stamp) {
+    const int L = c->L, X = c->X, Y = c->Y, XY = X * Y, ML = L + 1, LL = L * L;
+    const int SENT = LL + 1;
+    const int n = d.n;
+    const int64_t E = d.par_off[n];
+    std::vector<int64_t> csum(n + 1, 0);
+    for (int i = 0; i < n; i++) csum[i + 1] = csum[i] + d.child_num[i];
+    auto pos = [&](int i, int& x, int& y) { x = d.node[i] / Y % X; y = d.node[i] % Y; };
+    auto wseg = [&](int l, int xmin, int xmax, int ymin, int ymax) {
+        return c->presum[c->IDX(l, xmax, ymax)] - c->presum[c->IDX(l, xmin, ymin)];
+    };
+    // Edge list: edge k belongs to child node edge_node[k].
+    std::vector<int> edge_node(E);
+    for (int i = 0; i < n; i++)
+        for (int64_t k = d.par_off[i]; k < d.par_off[i + 1]; k++) edge_node[k] = i;
+    // precompute_cost_edge
+    std::vector<double> cost_edges((size_t)E * LL, INF);
+    std::vector<int> best_change((size_t)E * LL, -1);
+    for (int64_t e = 0; e < E; e++) {
+        int to = d.par[e], from = edge_node[e];
+        int tox, toy, fromx, fromy;
+        pos(to, tox, toy);
+        pos(from, fromx, fromy);
+        for (int fl = 0; fl < L; fl++) {
+            if ((c->dir(fl) == 0 && fromy != toy) || (c->dir(fl) == 1 && fromx != tox)) {
+                for (int tl = 0; tl < L; tl++) cost_edges[e * LL + fl * L + tl] = INF;
+                continue;
+            }
+            int distance = std::abs(tox - fromx) + std::abs(toy - fromy);
+            if (distance == 0) continue;
+            int dd = distance / std::min(distance, 20);
+            bool vert = fromx == tox;
+            int s0 = vert ? std::min(fromy, toy) : std::min(fromx, tox);
+            int s1 = vert ? std::max(fromy, toy) : std::max(fromx, tox);
+            for (int tl = 0; tl < L; tl++) {
+                if ((tl % 2) != (fl % 2)) continue;
+                double mn = INF;
+                int decide = -1;
+                for (int sp = s0 + dd; sp < s1; sp += dd) {
+                    int sx = vert ? fromx : sp, sy = vert ? sp : fromy;
+                    double via = 0;
+                    for (int ll = std::min(fl, tl); ll < std::max(fl, tl); ll++) via += (double)c->vcost[c->IDX(ll, sx, sy)];
+                    double cost = via;
+                    if (vert)
+                        cost += (wseg(fl, fromx, fromx, std::min(fromy, sy), std::max(fromy, sy)) +
+                                 wseg(tl, tox, tox, std::min(toy, sy), std::max(toy, sy)));
+                    else
+                        cost += (wseg(fl, std::min(fromx, sx), std::max(fromx, sx), fromy, fromy) +
+                                 wseg(tl, std::min(tox, sx), std::max(tox, sx), toy, toy));
+                    if (cost < mn) { mn = cost; decide = sp; }
+                }
+                cost_edges[e * LL + fl * L + tl] = mn;
+                best_change[e * LL + fl * L + tl] = decide;
+            }
+        }
+    }
+    // Bottom-up DP by decreasing depth.
+    std::vector<double> costs((size_t)n * ML, INF);
+    std::vector<uint64_t> packed((size_t)csum[n] * ML, INIT_KEY), best((size_t)csum[n] * ML, 0);
+    std::vector<int> byd(n);
+    for (int i = 0; i < n; i++) byd[i] = i;
+    std::stable_sort(byd.begin(), byd.end(), [&](int a, int b) { return d.depth[a] > d.depth[b]; });
+    std::vector<double> mcc;
+    std::vector<uint64_t> bp;
+    for (int node : byd) {
+        int x, y;
+        pos(node, x, y);
+        const int fixed_low = 1 + d.node[node] / XY;
+        const int fixed_high = fixed_low == ML ? 0 : fixed_low;
+        const int nc = d.child_num[node];
+        uint64_t* cur_best = &best[csum[node] * ML];
+        const uint64_t* cur_packed = &packed[csum[node] * ML];
+        double* cst = &costs[(size_t)node * ML];
+        mcc.assign(nc, INF);
+        bp.assign(nc, 0);
+        for (int low = 1; low <= fixed_low; low++) {
+            for (int k = 0; k < nc; k++) mcc[k] = INF;
+            double via = 0;
+            for (int li = low; li < L + 1; li++) {
+                if (li > low) via += (double)c->vcost[c->IDX(li - 2, x, y)];
+                for (int k = 0; k < nc; k++) {
+                    double cc;
+                    uint32_t id;
+                    unpack_key(cur_packed[k * ML + li], &cc, &id);
+                    if (cc < mcc[k]) { mcc[k] = cc; bp[k] = (uint64_t)id * ML + li; }
+                }
+                if (li >= fixed_high) {
+                    double cost = via;
+                    for (int k = 0; k < nc; k++) cost += mcc[k];
+                    if (cost < INF && cost < cst[li]) {
+                        cst[li] = cost;
+                        for (int k = 0; k < nc; k++) cur_best[k * ML + li] = bp[k];
+                    }
+                }
+            }
+            for (int li = L + 1 - 2; li >= low; li--)
+                if (cst[li + 1] < cst[li]) {
+                    cst[li] = cst[li + 1];
+                    for (int k = 0; k < nc; k++) cur_best[k * ML + li] = cur_best[k * ML + li + 1];
+                }
+        }
+        for (int64_t e = d.par_off[node]; e < d.par_off[node + 1]; e++) {
+            int p = d.par[e], ci = d.slot[e];
+            uint64_t* ppk = &packed[csum[p] * ML];
+            int px, py;
+            pos(p, px, py);
+            int minx = std::min(x, px), maxx = std::max(x, px), miny = std::min(y, py), maxy = std::max(y, py);
+            std::vector<uint64_t> cache(ML, 0);
+            for (int t = 1; t < ML; t++) cache[t] = ppk[ci * ML + t];
+            for (int layer = 1; layer < ML; layer++) {
+                if (((layer & 1) ^ c->DIR) == 1 && y != py) continue;
+                if (((layer & 1) ^ c->DIR) == 0 && x != px) continue;
+                double base = cst[layer];
+                for (int t = 1 + (1 - layer % 2); t < ML; t += 2) {
+                    double cost2;
+                    int choice = SENT;
+                    if (t == layer) cost2 = base + wseg(layer - 1, minx, maxx, miny, maxy);
+                    else if (std::abs(x - px) + std::abs(y - py) > 4) {
+                        int q = (layer - 1) * L + (t - 1);
+                        double ce = cost_edges[e * LL + q];
+                        if (ce < INF) { cost2 = base + ce; choice = q; }
+                        else continue;
+                    } else continue;
+                    uint64_t key = pack_key((float)cost2, (uint32_t)node * ID_MUL + (uint32_t)choice);
+                    if (key < cache[t]) cache[t] = key;
+                }
+            }
+            for (int t = 1; t < ML; t++) ppk[ci * ML + t] = std::min(ppk[ci * ML + t], cache[t]);
+        }
+    }
+    // Top-down extraction by increasing depth.
+    // Route pieces are collected per node (parent edge first, then the via
+    // stack) and concatenated in node order, one fixed order for the
+    // reference's racy append.
+    std::vector<int> lo(n, -1);
+    std::vector<std::vector<int>> em(n);
+    int cur_node = 0;
+    auto push = [&](int a, int b) { em[cur_node].push_back(a); em[cur_node].push_back(b); };
+    for (auto it = byd.rbegin(); it != byd.rend(); ++it) {
+        const int node = *it;
+        cur_node = node;
+        int cx, cy;
+        pos(node, cx, cy);
+        const int l = d.node[node] / XY;
+        const uint64_t* cur_best = &best[csum[node] * ML];
+        if (d.par_off[node + 1] == d.par_off[node]) {
+            int min_layer = 0;
+            double min_cost = costs[(size_t)node * ML];
+            for (int layer = 1; layer < ML; layer++)
+                if (costs[(size_t)node * ML + layer] < min_cost) { min_cost = costs[(size_t)node * ML + layer]; min_layer = layer; }
+            lo[node] = min_layer;
+        } else {
+            int par_layer = -1, par_idx = -1;
+            int64_t par_edge = -1;
+            for (int64_t e = d.par_off[node]; e < d.par_off[node + 1]; e++) {
+                int pn = d.par[e];
+                if (lo[pn] > 0) {
+                    par_idx = pn;
+                    par_edge = e;
+                    par_layer = lo[pn];
+                    uint64_t path = best[(csum[pn] + d.slot[e]) * ML + par_layer];
+                    uint64_t child = path / ML / ID_MUL, choice = (path / ML) % ID_MUL;
+                    if (child == (uint64_t)node) {
+                        lo[node] = choice == (uint64_t)SENT ? (int)(path % ML) : (int)(choice / L) + 1;
+                        int px, py;
+                        pos(pn, px, py);
+                        if (px == cx && cy != py) {
+                            if (choice == (uint64_t)SENT) {
+                                int ly = lo[node] - 1;
+                                add_wire_segment(c, ly, px, px, std::min(py, cy), std::max(py, cy), stamp);
+                                push((int)c->IDX(ly, px, std::min(py, cy)), (int)c->IDX(ly, px, std::max(py, cy)));
+                            } else {
+                                int fl = (int)choice / L, tl = (int)choice % L;
+                                int ch = best_change[e * LL + fl * L + tl];
+                                add_wire_segment(c, fl, cx, cx, std::min(cy, ch), std::max(cy, ch), stamp);
+                                add_wire_segment(c, tl, cx, cx, std::min(py, ch), std::max(py, ch), stamp);
+                                push((int)c->IDX(fl, cx, std::min(cy, ch)), (int)c->IDX(fl, cx, std::max(cy, ch)));
+                                push((int)c->IDX(std::min(fl, tl), cx, ch), (int)c->IDX(std::max(fl, tl), cx, ch));
+                                push((int)c->IDX(tl, cx, std::min(py, ch)), (int)c->IDX(tl, cx, std::max(py, ch)));
+                            }
+                        } else if (py == cy && cx != px) {
+                            if (choice == (uint64_t)SENT) {
+                                int ly = lo[node] - 1;
+                                add_wire_segment(c, ly, std::min(px, cx), std::max(px, cx), py, py, stamp);
+                                push((int)c->IDX(ly, std::min(px, cx), py), (int)c->IDX(ly, std::max(px, cx), py));
+                            } else {
+                                int fl = (int)choice / L, tl = (int)choice % L;
+                                int ch = best_change[e * LL + fl * L + tl];
+                                add_wire_segment(c, fl, std::min(cx, ch), std::max(cx, ch), cy, cy, stamp);
+                                add_wire_segment(c, tl, std::min(px, ch), std::max(px, ch), cy, cy, stamp);
+                                push((int)c->IDX(fl, std::min(cx, ch), cy), (int)c->IDX(fl, std::max(cx, ch), cy));
+                                push((int)c->IDX(std::min(fl, tl), ch, cy), (int)c->IDX(std::max(fl, tl), ch, cy));
+                                push((int)c->IDX(tl, std::min(px, ch), cy), (int)c->IDX(tl, std::max(px, ch), cy));
+                            }
+                        }
+                        break;
+                    } else {
+                        lo[node] = -1;
+                    }
+                }
+            }
+            if (par_layer == -1) { lo[node] = -1; continue; }
+            uint64_t path = best[(csum[par_idx] + d.slot[par_edge]) * ML + par_layer];
+            if (path / ML / ID_MUL != (uint64_t)node) { lo[node] = -1; continue; }
+        }
+        // A source without any finite layer has no route (the reference would
+        // index layer -1); children without a recorded layer are skipped.
+        if (lo[node] < 1) continue;
+        int minl = l + 1;
+        int maxl = (l + 1) == ML ? 1 : minl;
+        minl = std::min(minl, lo[node]);
+        maxl = std::max(maxl, lo[node]);
+        for (int k = 0; k < d.child_num[node]; k++) {
+            int lc = (int)(cur_best[k * ML + lo[node]] % ML);
+            if (lc < 1) continue;
+            minl = std::min(lc, minl);
+            maxl = std::max(lc, maxl);
+        }
+        if (minl < maxl) push((int)c->IDX(minl - 1, cx, cy), (int)c->IDX(maxl - 1, cx, cy));
+    }
+    r.assign(1, 1);
+    for (int i = 0; i < n; i++) r.insert(r.end(), em[i].begin(), em[i].end());
+    r[0] = (int)r.size();
+}
+
+// Scheduler core shared by the pattern pass and the DAG pass: points come from
+// pt_off/pt per listed net when given, from the pattern footprint otherwise.
+int schedule_impl(orc_ctx* c, const int* order, int64_t n, int mode, const int64_t* pt_off, const int* pt,
+                  int* batch_of);
+
+} // namespace
+
+int orc_dag_pass(orc_ctx* c, const int* nets, int64_t n, int mode, const int64_t* node_off, const int* node,
+                 const int64_t* par_off, const int* par, const int* slot, const int* child_num, const int* depth,
+                 const int64_t* pt_off, const int* pt) {
+    if (n == 0) return 0;
+    std::vector<int> batch_of(n);
+    int nb = schedule_impl(c, nets, n, mode, pt_off, pt, batch_of.data());
+    std::vector<std::vector<int64_t>> batches(nb);  // list positions
+    for (int64_t i = n - 1; i >= 0; i--) batches[batch_of[i]].push_back(i);
+    for (int b = nb - 1; b >= 0; b--) {
+        const auto& B = batches[b];
+        int s1 = ++c->stamp;
+        for (int64_t i : B) commit_wire_demand(c, c->routes[nets[i]], s1, -1);
+        for (int64_t i : B) commit_via_demand(c, c->routes[nets[i]], s1, -1);
+        int s2 = ++c->stamp;
+        orc_update_cost(c);
+        orc_compute_presum(c);
+        for (int64_t i : B) {
+            int64_t b0 = node_off[i];
+            std::vector<int64_t> poff(node_off[i + 1] - b0 + 1);
+            for (size_t k = 0; k < poff.size(); k++) poff[k] = par_off[b0 + k] - par_off[b0];
+            DagView v{(int)(node_off[i + 1] - b0), node + b0, poff.data(), par + par_off[b0], slot + par_off[b0],
+                      child_num + b0, depth + b0};
+            dag_route_net(c, v, c->routes[nets[i]], s2);
+        }
+        batch_wire_update(c, s2);
+        for (int64_t i : B) commit_via_demand(c, c->routes[nets[i]], s2, 1);
+    }
+    return nb;
 }

@@ -69,6 +69,16 @@ void orc_ripup_mark(orc_ctx* c, int threshold, uint8_t* is_of);
 /* Congestion view (src/graph.hpp:545-584) with zeroed start. */
 void orc_congestion_view(orc_ctx* c, float* xsum, float* ysum);
 
+/* One rip-up-and-reroute pass over detour DAGs (src/Lshape_route_detour.hpp:
+ * 726-1112): 'nets' in routing priority order (highest first); DAG i belongs
+ * to nets[i] (node_off/pt_off per listed net, par_off global over nodes, par
+ * local).  Schedules with the DAG points, then per batch (last first) rips up
+ * the nets' current routes, recomputes costs, routes the DAGs and commits.
+ * Returns the number of batches. */
+int orc_dag_pass(orc_ctx* c, const int* nets, int64_t n, int mode, const int64_t* node_off, const int* node,
+                 const int64_t* par_off, const int* par, const int* slot, const int* child_num, const int* depth,
+                 const int64_t* pt_off, const int* pt);
+
 /* Single-net L-shape DP exposed for brute-force checks: routes the tree of
  * net 'net' against the current costs without committing; returns the optimal
  * root cost and writes the route. */

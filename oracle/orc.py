@@ -180,3 +180,23 @@ class Oracle:
     @property
     def stamp(self):
         return lib().orc_stamp(self.h)
+
+
+def _dag_bind():
+    L = lib()
+    if not getattr(L, "_dag_bound", False):
+        L.orc_dag_pass.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int] + [C.c_void_p] * 9
+        L.orc_dag_pass.restype = C.c_int
+        L._dag_bound = True
+    return L
+
+
+def dag_pass(o, nets, dags, mode=0):
+    """Run one rip-up-and-reroute pass of Oracle o over the DAGs of 'nets'
+    (the dict returned by instantgr_amd.host.Design.build_dags)."""
+    L = _dag_bind()
+    nets = np.ascontiguousarray(nets, np.int32)
+    keys = ("dag_node_off", "dag_node", "dag_par_off", "dag_par", "dag_slot", "dag_child_num", "dag_depth",
+            "dag_pt_off", "dag_pt")
+    arrs = [np.ascontiguousarray(dags[k]) for k in keys]
+    return L.orc_dag_pass(o.h, nets.ctypes.data, len(nets), mode, *[a.ctypes.data for a in arrs])

@@ -1,0 +1,58 @@
+"""GPU parity of the rip-up-and-reroute passes: three passes after the
+pattern pass on benchmark configuration 0, every route and every demand
+value bit-identical to the CPU oracle, same evaluator score."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import orc
+from instantgr_amd import gpu
+
+pytestmark = pytest.mark.gpu
+
+
+def _routes(h):
+    off, data = h.routes_flat()
+    return [data[off[i]:off[i] + data[off[i]]] for i in range(len(off) - 1)]
+
+
+def test_rrr_three_passes_c1(design_c1):
+    d = design_c1
+    r = gpu.Router.from_design(d)
+    o = orc.Oracle.from_design(d)
+    r.set_nets_from_design(d)
+    o.set_nets_from_design(d)
+    order = d.array("order")
+    r.pattern_route(order)
+    o.pattern_route(order)
+    hp = d.array("hpwl")
+    for it in range(3):
+        xg, yg = r.congestion_view()
+        xo, yo = o.congestion_view()
+        assert np.array_equal(xg, xo) and np.array_equal(yg, yo)
+        ig = r.ripup_mark(0)
+        assert np.array_equal(ig, o.ripup_mark(0))
+        nets = np.nonzero(ig)[0].astype(np.int32)
+        nets = nets[np.argsort(-hp[nets], kind="stable")]
+        dg = d.build_dags(nets, xg, yg, 0, 8)
+        st = r.dag_pass(nets, dg)
+        nb = orc.dag_pass(o, nets, dg)
+        assert st["batches"] == nb, it
+        assert np.array_equal(r.demand().view(np.uint32), o.demand().view(np.uint32)), it
+        assert np.array_equal(r.timestamp(), o.timestamp()), it
+        rg = _routes(r)
+        ro = [o.route(i) for i in range(o.num_nets)]
+        for i in range(len(rg)):
+            assert np.array_equal(rg[i], ro[i]), (it, i)
+    with tempfile.TemporaryDirectory() as tmp:
+        scores = []
+        for h in (r, o):
+            off, data = (h.routes_flat() if h is r else o.routes_flat())
+            p = os.path.join(tmp, "out")
+            d.write_routes(p, off, data)
+            scores.append(d.evaluate(p))
+        assert scores[0]["opens"] == 0
+        assert scores[0]["total"] == scores[1]["total"]
+    r.close()
