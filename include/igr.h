@@ -118,6 +118,31 @@ typedef struct igr_dag_desc {
 int igr_dag_pass(igr_ctx* ctx, const int* nets, int64_t n, int mode, int form, const igr_dag_desc* dags,
                  igr_pass_stats* stats);
 
+#define IGR_MAX_RRR 8
+
+typedef struct igr_run_opts {
+    int form;                     /* IGR_FORM_PLAIN or IGR_FORM_BATCHED */
+    int rrr_iterations;           /* rip-up-and-reroute passes after the pattern pass */
+    int threads;                  /* host threads for the detour DAGs */
+} igr_run_opts;
+
+typedef struct igr_run_stats {
+    double upload_s, pattern_s, pattern_schedule_s, total_s;
+    int pattern_batches, iterations;
+    int64_t subnets;
+    int64_t ripped[IGR_MAX_RRR];
+    int rrr_batches[IGR_MAX_RRR];
+    int64_t rrr_dag_nodes[IGR_MAX_RRR];
+    double rrr_view_s[IGR_MAX_RRR], rrr_dag_build_s[IGR_MAX_RRR], rrr_route_s[IGR_MAX_RRR];
+} igr_run_stats;
+
+struct igrh_design;
+/* The routing stage end to end (src/main.cpp:22-73) on a design whose trees
+ * igrh_build has prepared (include/igr_host.h): nets upload, pattern pass,
+ * then rip-up and reroute passes until 'rrr_iterations' or no net overflows.
+ * Routes stay in the context (igr_get_routes). */
+int igr_run(igr_ctx* ctx, struct igrh_design* design, const igr_run_opts* opts, igr_run_stats* stats);
+
 /* Routes: r[0] = next free slot (1 = empty), then pairs of cell indices. */
 int igr_get_route(igr_ctx* ctx, int net, int* buf, int cap, int* len);
 /* All routes packed: off[n+1] (host) where off[i+1]-off[i] is net i's slot
@@ -131,6 +156,15 @@ int igr_ripup_mark(igr_ctx* ctx, int threshold, uint8_t* is_of);
 /* graph::extract_congestionView{,_xsum,_ysum} (src/graph.hpp:545-584):
  * X*Y prefix views indexed x*Y + y. */
 int igr_congestion_view(igr_ctx* ctx, float* xsum, float* ysum);
+
+/* Device time per kernel class, measured with events on the context's stream
+ * while enabled (enabling clears earlier records).  Classes: 0 cost update,
+ * 1 prefix sums, 2 L-shape DP, 3 wire commit, 4 via commit, 5 DAG edge
+ * layer changes, 6 DAG DP, 7 DAG extraction, 8 scheduler, 9 fused track
+ * update, 10 rip-up.  ms and launches have IGR_KCLASSES entries. */
+#define IGR_KCLASSES 11
+int igr_set_profiling(igr_ctx* ctx, int on);
+int igr_kernel_times(igr_ctx* ctx, double* ms, int64_t* launches);
 
 /* Current commit timestamp counter. */
 int igr_stamp(igr_ctx* ctx);

@@ -205,13 +205,10 @@ __global__ void __launch_bounds__(TPB) k_dag_dp(DagArgs a, const int* lvl_nodes,
     }
 }
 
-// Top-down extraction of one node of the current depth level.
-__global__ void __launch_bounds__(TPB) k_dag_extract(DagArgs a, const int* lvl_nodes, int64_t lo_i, int64_t hi_i) {
-    int64_t t = lo_i + (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (t >= hi_i) return;
+// Top-down extraction of one node whose parents are all decided.
+__device__ void extract_node(const DagArgs& a, int node) {
     const GridDev& g = a.g;
     const int L = g.L, ML = L + 1, LL = L * L, SENT = LL + 1, XY = g.X * g.Y;
-    const int node = lvl_nodes[t];
     const int nr = (int)(node - a.node0);
     const int cx = a.node[node] / g.Y % g.X, cy = a.node[node] % g.Y;
     const int l = a.node[node] / XY;
@@ -311,6 +308,13 @@ __global__ void __launch_bounds__(TPB) k_dag_extract(DagArgs a, const int* lvl_n
     em[0] = ne;
 }
 
+// Plain form: one launch per depth level, one thread per node.
+__global__ void __launch_bounds__(TPB) k_dag_extract(DagArgs a, const int* lvl_nodes, int64_t lo_i, int64_t hi_i) {
+    int64_t t = lo_i + (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= hi_i) return;
+    extract_node(a, lvl_nodes[t]);
+}
+
 // Concatenate the node slots of each net in node order into its new route and
 // point the net at it.
 __global__ void __launch_bounds__(TPB) k_dag_compact(const int* nets, int64_t net0, int64_t net1,
@@ -330,6 +334,266 @@ __global__ void __launch_bounds__(TPB) k_dag_compact(const int* nets, int64_t ne
 }
 
 inline unsigned blocks(int64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+// ---------------------------------------------------------------------------
+// Batched form: one 64-lane workgroup routes one net's whole DAG, walking its
+// depth levels with workgroup barriers instead of one launch per level; all
+// nets of the batch run concurrently.  Per level the node DP runs one lane
+// per node, then the parent-edge candidates run one lane per (edge, layer)
+// and go straight to the parent slots with 64-bit atomicMin (the minimum
+// over all candidates is the same as the reference's per-thread cache
+// followed by one atomicMin).  Extraction walks the levels top-down, then the
+// lanes concatenate the node slots into the route with a wave prefix sum.
+// ---------------------------------------------------------------------------
+struct NetLevels {
+    const int64_t* nl_off;     // per net: first entry in lvl_node_off / lvl_edge_off
+    const int* nlevels;        // per net: number of depth levels
+    const int64_t* lvl_node_off;  // per level: range in dnode
+    const int64_t* lvl_edge_off;  // per level: range in dedge
+    const int* dnode;          // nodes grouped by depth within each net
+    const int64_t* dedge;      // edges of each level's nodes
+};
+
+constexpr int WTPB = 64;
+
+// Node DP with 16 lanes per node, lane s holding layer index s (1..L).  Same
+// results as dp_node: for each bottom layer 'low' the running minimum of every
+// child slot over layers [low, s] is an inclusive prefix-min across the lanes
+// (ties keep the lower layer, as the sequential strict '<' does), the via sum
+// is accumulated in layer order inside the lane, and the downward sweep is a
+// suffix argmin across the lanes (ties keep the lower layer).  'valid' is
+// false on lane groups without a node; every lane of the wave calls this.
+__device__ void dp_node16(const DagArgs& a, int node, bool valid, int ncw) {
+    const GridDev& g = a.g;
+    const int L = g.L, ML = L + 1, XY = g.X * g.Y;
+    const int lane = threadIdx.x & 63, s = lane & 15;
+    int nc = 0, fixed_low = 0, fixed_high = 0, x = 0, y = 0, nr = 0;
+    int64_t cs = 0;
+    if (valid) {
+        nr = (int)(node - a.node0);
+        x = a.node[node] / g.Y % g.X;
+        y = a.node[node] % g.Y;
+        fixed_low = 1 + a.node[node] / XY;
+        fixed_high = fixed_low == ML ? 0 : fixed_low;
+        nc = a.child_num[node];
+        cs = a.csum[node] - a.slot0;
+    }
+    const bool act = valid && s >= 1 && s <= L;
+    double ck[10];
+    uint32_t idk[10];
+#pragma unroll
+    for (int k = 0; k < 10; k++) {
+        ck[k] = INF;
+        idk[k] = 0;
+        if (k < nc && act) {
+            const uint64_t key = a.packed[(cs + k) * ML + s];
+            ck[k] = unpack_cost(key);
+            idk[k] = (uint32_t)key;
+        }
+    }
+    float v[MAXL];
+#pragma unroll
+    for (int t = 0; t < MAXL; t++) v[t] = (valid && t < L) ? g.vcost[gidx(g, t, x, y)] : 0.0f;
+    double cst = INF;
+    uint64_t bp[10];
+#pragma unroll
+    for (int k = 0; k < 10; k++) bp[k] = 0;
+    // Lowest layer: group-uniform bound, the wave runs to its largest.
+    int lowmax = fixed_low;
+    for (int d = 16; d < 64; d <<= 1) lowmax = max(lowmax, __shfl_xor(lowmax, d, 64));
+    for (int low = 1; low <= lowmax; low++) {
+        const bool run = act && low <= fixed_low;
+        double via = 0;
+#pragma unroll
+        for (int t = 2; t <= MAXL; t++)
+            if (t > low && t <= s) via += (double)v[t - 2];
+        double cost = via;
+        uint64_t cb[10];
+#pragma unroll
+        for (int k = 0; k < 10; k++) {
+            cb[k] = 0;
+            if (k >= ncw) continue;
+            double m = (run && s >= low && k < nc && ck[k] < INF) ? ck[k] : INF;
+            uint64_t b = (uint64_t)idk[k] * ML + s;
+#pragma unroll
+            for (int d = 1; d < 16; d <<= 1) {
+                const double om = __shfl_up(m, d, 16);
+                const uint64_t ob = __shfl_up(b, d, 16);
+                if (s >= d && !(m < om)) { m = om; b = ob; }
+            }
+            cb[k] = b;
+            if (k < nc) cost += m;
+        }
+        if (run && s >= low && s >= fixed_high && cost < INF && cost < cst) {
+            cst = cost;
+#pragma unroll
+            for (int k = 0; k < 10; k++) bp[k] = cb[k];
+        }
+        // Suffix argmin over lanes [low, L]: lane s takes the smallest value
+        // above it only when strictly smaller.
+        double c = (valid && s >= low && s <= L) ? cst : __longlong_as_double(0x7ff0000000000000LL);
+        int src = s;
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            const double oc = __shfl_down(c, d, 16);
+            const int os = __shfl_down(src, d, 16);
+            if (s + d < 16 && oc < c) { c = oc; src = os; }
+        }
+        const double nc_cost = __shfl(cst, src, 16);
+#pragma unroll
+        for (int k = 0; k < 10; k++) {
+            if (k >= ncw) continue;
+            const uint64_t nb = __shfl(bp[k], src, 16);
+            if (run && s >= low && src != s) bp[k] = nb;
+        }
+        if (run && s >= low && src != s) cst = nc_cost;
+    }
+    if (valid && s <= L) {
+        double* cstp = a.costs + (int64_t)nr * ML;
+        cstp[s] = s == 0 ? INF : cst;
+        if (s >= 1)
+            for (int k = 0; k < nc; k++) a.best[(cs + k) * ML + s] = bp[k];
+    }
+}
+
+// Candidates of one parent edge from one source layer of the child.
+__device__ void dp_edge_layer(const DagArgs& a, int64_t e, int layer) {
+    const GridDev& g = a.g;
+    const int L = g.L, ML = L + 1, LL = L * L, SENT = LL + 1;
+    const int node = a.enode[e];
+    const int nr = (int)(node - a.node0);
+    const int x = a.node[node] / g.Y % g.X, y = a.node[node] % g.Y;
+    const int p = a.par[e], ci = a.slot[e];
+    const int px = a.node[p] / g.Y % g.X, py = a.node[p] % g.Y;
+    if (((layer & 1) ^ g.DIR) == 1 && y != py) return;
+    if (((layer & 1) ^ g.DIR) == 0 && x != px) return;
+    uint64_t* ppk = a.packed + (a.csum[p] - a.slot0 + ci) * ML;
+    const int minx = min(x, px), maxx = max(x, px), miny = min(y, py), maxy = max(y, py);
+    const int64_t er = e - a.edge0;
+    const uint32_t myid = (uint32_t)a.loc[node] * ID_MUL;
+    const double base = a.costs[(int64_t)nr * ML + layer];
+    for (int tt = 1 + (1 - layer % 2); tt < ML; tt += 2) {
+        double cost2;
+        int choice = SENT;
+        if (tt == layer) cost2 = base + wseg(g, layer - 1, minx, maxx, miny, maxy);
+        else if (abs(x - px) + abs(y - py) > 4) {
+            const int q = (layer - 1) * L + (tt - 1);
+            const double ce = a.cost_edges[er * LL + q];
+            if (ce < INF) { cost2 = base + ce; choice = q; }
+            else continue;
+        } else continue;
+        atomicMin((unsigned long long*)&ppk[tt], (unsigned long long)pack_key((float)cost2, myid + (uint32_t)choice));
+    }
+}
+
+// Batched form of precompute_cost_edge: one thread per (edge, from layer,
+// to layer), computing only the entries the DP reads (to layer of the same
+// direction and different from the from layer, from layer along the edge,
+// edges longer than 4 gcells).  Each entry is the same strict first minimum
+// over the same sampled positions as the reference's.
+__global__ void __launch_bounds__(TPB) k_dag_edges_pairs(DagArgs a, int64_t edges) {
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    const GridDev& g = a.g;
+    const int L = g.L, LL = L * L;
+    if (t >= edges * LL) return;
+    const int64_t er = t / LL;
+    const int q = (int)(t % LL), fl = q / L, tl = q % L;
+    if (tl == fl || (tl % 2) != (fl % 2)) return;
+    const int64_t e = a.edge0 + er;
+    const int to = a.par[e], from = a.enode[e];
+    const int tox = a.node[to] / g.Y % g.X, toy = a.node[to] % g.Y;
+    const int fromx = a.node[from] / g.Y % g.X, fromy = a.node[from] % g.Y;
+    if ((layer_dir(fl, g.DIR) == 0 && fromy != toy) || (layer_dir(fl, g.DIR) == 1 && fromx != tox)) return;
+    const int distance = abs(tox - fromx) + abs(toy - fromy);
+    if (distance <= 4) return;
+    const int dd = distance / min(distance, 20);
+    const bool vert = fromx == tox;
+    const int s0 = vert ? min(fromy, toy) : min(fromx, tox);
+    const int s1 = vert ? max(fromy, toy) : max(fromx, tox);
+    const int lo = min(fl, tl), hi = max(fl, tl);
+    double mn = INF;
+    int decide = -1;
+    for (int sp = s0 + dd; sp < s1; sp += dd) {
+        const int sx = vert ? fromx : sp, sy = vert ? sp : fromy;
+        double via = 0;
+        for (int ll = lo; ll < hi; ll++) via += (double)g.vcost[gidx(g, ll, sx, sy)];
+        double cost = via;
+        if (vert)
+            cost += (wseg(g, fl, fromx, fromx, min(fromy, sy), max(fromy, sy)) +
+                     wseg(g, tl, tox, tox, min(toy, sy), max(toy, sy)));
+        else
+            cost += (wseg(g, fl, min(fromx, sx), max(fromx, sx), fromy, fromy) +
+                     wseg(g, tl, min(tox, sx), max(tox, sx), toy, toy));
+        if (cost < mn) { mn = cost; decide = sp; }
+    }
+    a.cost_edges[er * LL + q] = mn;
+    a.best_change[er * LL + q] = decide;
+}
+
+__global__ void __launch_bounds__(WTPB) k_dag_net(DagArgs a, NetLevels lv, const int* nets, int64_t net0, int64_t net1,
+                                                  const int64_t* node_off, const int64_t* route_slot, int* routes,
+                                                  int** rptr) {
+    const int64_t i = net0 + blockIdx.x;
+    if (i >= net1) return;
+    const int lane = threadIdx.x;
+    const GridDev& g = a.g;
+    const int L = g.L, ML = L + 1;
+    const int64_t n0 = node_off[i], n1 = node_off[i + 1];
+    const int64_t e0 = a.par_off[n0], e1 = a.par_off[n1];
+    const int64_t s0 = a.csum[n0], s1 = a.csum[n1];
+    // Initial values (src/Lshape_route_detour.hpp:75-121) and edge layer changes.
+    for (int64_t k = lane; k < (n1 - n0) * ML; k += WTPB) a.costs[(n0 - a.node0) * ML + k] = INF;
+    for (int64_t k = lane; k < (s1 - s0) * ML; k += WTPB) {
+        a.packed[(s0 - a.slot0) * ML + k] = INIT_KEY;
+        a.best[(s0 - a.slot0) * ML + k] = 0;
+    }
+    __syncthreads();
+    const int nlev = lv.nlevels[i];
+    const int64_t lb = lv.nl_off[i];
+    for (int d = nlev - 1; d >= 0; d--) {
+        const int64_t a0 = lv.lvl_node_off[lb + d], a1 = lv.lvl_node_off[lb + d + 1];
+        for (int64_t p = a0; p < a1; p += 4) {
+            const int64_t k = p + (lane >> 4);
+            const bool valid = k < a1;
+            const int node = valid ? lv.dnode[k] : 0;
+            int ncw = valid ? a.child_num[node] : 0;
+            for (int o = 16; o < 64; o <<= 1) ncw = max(ncw, __shfl_xor(ncw, o, 64));
+            dp_node16(a, node, valid, ncw);
+        }
+        __syncthreads();
+        const int64_t b0 = lv.lvl_edge_off[lb + d], b1 = lv.lvl_edge_off[lb + d + 1];
+        for (int64_t k = lane; k < (b1 - b0) * L; k += WTPB) dp_edge_layer(a, lv.dedge[b0 + k / L], (int)(k % L) + 1);
+        __syncthreads();
+        // The parent slots were updated by atomics at L2: drop stale L1 lines
+        // before the next level reads them.
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    for (int d = 0; d < nlev; d++) {
+        const int64_t a0 = lv.lvl_node_off[lb + d], a1 = lv.lvl_node_off[lb + d + 1];
+        for (int64_t k = a0 + lane; k < a1; k += WTPB) extract_node(a, lv.dnode[k]);
+        __syncthreads();
+    }
+    // Concatenate node slots in node order (wave prefix sum of the counts).
+    int* r = routes + route_slot[i];
+    int base = 1;
+    for (int64_t c0 = n0; c0 < n1; c0 += WTPB) {
+        const int64_t nd = c0 + lane;
+        const int* em = a.emit + (nd - a.node0) * EMIT;
+        const int cnt = nd < n1 ? em[0] : 0;
+        int incl = cnt;
+        for (int off = 1; off < WTPB; off <<= 1) {
+            int v = __shfl_up(incl, off, WTPB);
+            if (lane >= off) incl += v;
+        }
+        const int start = base + incl - cnt;
+        for (int k = 0; k < cnt; k++) r[start + k] = em[1 + k];
+        base += __shfl(incl, WTPB - 1, WTPB);
+    }
+    if (lane == 0) {
+        r[0] = base;
+        rptr[nets[i]] = r;
+    }
+}
 
 DagArgs make_args(igr_ctx* c, const DagBatch& b, int stamp) {
     DagDev& D = c->dag;
@@ -353,17 +617,43 @@ void launch_dag_batch_plain(igr_ctx* c, const DagBatch& b, int stamp, const std:
     HIP_OK(hipGetLastError());
     DagArgs a = make_args(c, b, stamp);
     if (ne > 0) {
+        KScope ks(c, K_DAG_EDGES);
         k_dag_edges<<<blocks(ne * c->g.L), TPB, 0, s>>>(a, ne);
         HIP_OK(hipGetLastError());
     }
-    for (int d = b.levels - 1; d >= 0; d--) {
-        int64_t lo = h_lvl_off[b.lvl0 + d], hi = h_lvl_off[b.lvl0 + d + 1];
-        if (hi > lo) k_dag_dp<<<blocks(hi - lo), TPB, 0, s>>>(a, D.lvl_nodes.p, lo, hi);
+    {
+        KScope ks(c, K_DAG_DP);
+        for (int d = b.levels - 1; d >= 0; d--) {
+            int64_t lo = h_lvl_off[b.lvl0 + d], hi = h_lvl_off[b.lvl0 + d + 1];
+            if (hi > lo) k_dag_dp<<<blocks(hi - lo), TPB, 0, s>>>(a, D.lvl_nodes.p, lo, hi);
+        }
     }
-    for (int d = 0; d < b.levels; d++) {
-        int64_t lo = h_lvl_off[b.lvl0 + d], hi = h_lvl_off[b.lvl0 + d + 1];
-        if (hi > lo) k_dag_extract<<<blocks(hi - lo), TPB, 0, s>>>(a, D.lvl_nodes.p, lo, hi);
+    {
+        KScope ks(c, K_DAG_EXTRACT);
+        for (int d = 0; d < b.levels; d++) {
+            int64_t lo = h_lvl_off[b.lvl0 + d], hi = h_lvl_off[b.lvl0 + d + 1];
+            if (hi > lo) k_dag_extract<<<blocks(hi - lo), TPB, 0, s>>>(a, D.lvl_nodes.p, lo, hi);
+        }
     }
+    HIP_OK(hipGetLastError());
+}
+
+void launch_dag_batch_wave(igr_ctx* c, const DagBatch& b, int stamp) {
+    DagDev& D = c->dag;
+    const int64_t n = b.net1 - b.net0;
+    if (n == 0) return;
+    DagArgs a = make_args(c, b, stamp);
+    const int64_t ne = b.edge1 - b.edge0;
+    if (ne > 0) {
+        KScope ke(c, K_DAG_EDGES);
+        const int64_t items = ne * c->g.L * c->g.L;
+        k_dag_edges_pairs<<<blocks(items), TPB, 0, c->stream>>>(a, ne);
+        HIP_OK(hipGetLastError());
+    }
+    NetLevels lv{D.nl_off.p, D.nlevels.p, D.lvl_node_off.p, D.lvl_edge_off.p, D.dnode.p, D.dedge.p};
+    KScope ks(c, K_DAG_DP);
+    k_dag_net<<<(unsigned)n, WTPB, 0, c->stream>>>(a, lv, D.nets.p, b.net0, b.net1, D.node_off.p, D.route_slot.p,
+                                                    c->nets.dag_routes.back()->p, c->nets.rptr.p);
     HIP_OK(hipGetLastError());
 }
 

@@ -269,6 +269,7 @@ int tracks(const GridDev& g) {
     } while (0)
 
 void launch_update_cost(igr_ctx* c) {
+    KScope ks(c, K_COST);
     int64_t n = c->cells();
     k_update_wcost<<<blocks(n), TPB, 0, c->stream>>>(c->g);
     HIP_OK(hipGetLastError());
@@ -277,10 +278,12 @@ void launch_update_cost(igr_ctx* c) {
 }
 
 void launch_presum(igr_ctx* c) {
+    KScope ks(c, K_PRESUM);
     DISPATCH_K(std::max(c->g.X, c->g.Y), k_presum, tracks(c->g), c->stream, c->g);
 }
 
 void launch_wire_commit(igr_ctx* c, int stamp) {
+    KScope ks(c, K_WIRE_COMMIT);
     DISPATCH_K(std::max(c->g.X, c->g.Y), k_scan_int, tracks(c->g), c->stream, c->g, c->g.pre);
     int64_t n = c->cells();
     k_commit_edges<<<blocks(n), TPB, 0, c->stream>>>(c->g, stamp);
@@ -290,6 +293,7 @@ void launch_wire_commit(igr_ctx* c, int stamp) {
 
 void launch_via_commit(igr_ctx* c, const int* d_nets, int64_t n, int stamp, int K) {
     if (n == 0) return;
+    KScope ks(c, K_VIA_COMMIT);
     k_via_commit<<<blocks(n), TPB, 0, c->stream>>>(c->g, d_nets, n, c->nets.rptr.p, stamp, K);
     HIP_OK(hipGetLastError());
 }
@@ -319,6 +323,7 @@ __global__ void __launch_bounds__(TPB) k_rip_wire(GridDev g, const int* nets, in
 
 void launch_rip_wire(igr_ctx* c, const int* d_nets, int64_t n, int stamp) {
     if (n == 0) return;
+    KScope ks(c, K_RIP);
     k_rip_wire<<<blocks(n), TPB, 0, c->stream>>>(c->g, d_nets, n, c->nets.rptr.p, stamp);
     HIP_OK(hipGetLastError());
 }

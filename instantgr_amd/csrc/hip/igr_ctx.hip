@@ -6,7 +6,9 @@
 #include <chrono>
 #include <cstring>
 #include <exception>
+#include <functional>
 #include <string>
+#include <thread>
 
 using namespace igr_dev;
 
@@ -396,73 +398,139 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
 
         // Routing order: batch nb-1 first, inside a batch decreasing list
         // position (the scheduler's index order).
-        std::vector<std::vector<int64_t>> bl(nb);
-        for (int64_t i = n - 1; i >= 0; i--) bl[batch_of[i]].push_back(i);
-        std::vector<int> rnets;
-        std::vector<int64_t> node_off(1, 0), par_off(1, 0), csum(1, 0), route_slot;
-        std::vector<int> node, par, slot, child_num, loc, enode, lvl_nodes;
-        std::vector<int64_t> lvl_off(1, 0);
-        std::vector<DagBatch> batches;
-        int64_t route_tot = 0;
-        int64_t mx_nodes = 0, mx_edges = 0, mx_slots = 0;
-        for (int b = nb - 1; b >= 0; b--) {
-            DagBatch B{};
-            B.net0 = (int64_t)rnets.size();
-            B.node0 = (int64_t)node.size();
-            B.edge0 = (int64_t)par.size();
-            B.slot0 = csum.back();
-            int maxd = 0;
-            for (int64_t i : bl[b]) {
-                const int64_t nb0 = dg->node_off[i], nb1 = dg->node_off[i + 1];
-                const int64_t g0 = (int64_t)node.size();
-                rnets.push_back(nets[i]);
-                route_slot.push_back(route_tot);
-                route_tot += 8 * (nb1 - nb0) + 1;
-                for (int64_t k = nb0; k < nb1; k++) {
-                    node.push_back(dg->nodes[k]);
-                    child_num.push_back(dg->child_num[k]);
-                    loc.push_back((int)(k - nb0));
-                    csum.push_back(csum.back() + dg->child_num[k]);
-                    maxd = std::max(maxd, dg->depth[k]);
-                    for (int64_t q = dg->par_off[k]; q < dg->par_off[k + 1]; q++) {
-                        par.push_back((int)(g0 + dg->par[q]));
-                        slot.push_back(dg->slot[q]);
-                        enode.push_back((int)(g0 + (k - nb0)));
-                    }
-                    par_off.push_back((int64_t)par.size());
-                }
-                node_off.push_back((int64_t)node.size());
+        std::vector<int64_t> cntb(nb + 1, 0);
+        for (int64_t i = 0; i < n; i++) cntb[nb - 1 - batch_of[i] + 1]++;
+        for (int b = 0; b < nb; b++) cntb[b + 1] += cntb[b];
+        std::vector<int64_t> rorder(n);
+        {
+            std::vector<int64_t> pos(cntb.begin(), cntb.end() - 1);
+            for (int64_t i = n - 1; i >= 0; i--) rorder[pos[nb - 1 - batch_of[i]]++] = i;
+        }
+        // Per routed net sizes, then bases, then a parallel fill.
+        std::vector<int64_t> nbase(n + 1, 0), ebase(n + 1, 0), sbase(n + 1, 0), lbase(n + 1, 0), route_slot(n + 1, 0);
+        const int nthreads = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+        auto pfor = [&](int64_t cnt, const std::function<void(int64_t, int64_t)>& fn) {
+            std::vector<std::thread> th;
+            int64_t chunk = (cnt + nthreads - 1) / nthreads;
+            for (int t = 0; t < nthreads; t++) {
+                int64_t a0 = t * chunk, a1 = std::min(cnt, a0 + chunk);
+                if (a0 < a1) th.emplace_back(fn, a0, a1);
             }
-            B.net1 = (int64_t)rnets.size();
-            B.node1 = (int64_t)node.size();
-            B.edge1 = (int64_t)par.size();
-            B.slot1 = csum.back();
+            for (auto& x : th) x.join();
+        };
+        pfor(n, [&](int64_t a0, int64_t a1) {
+            for (int64_t r = a0; r < a1; r++) {
+                const int64_t i = rorder[r], b0 = dg->node_off[i], b1 = dg->node_off[i + 1];
+                int64_t slots = 0;
+                int maxd = 0;
+                for (int64_t k = b0; k < b1; k++) { slots += dg->child_num[k]; maxd = std::max(maxd, dg->depth[k]); }
+                nbase[r + 1] = b1 - b0;
+                ebase[r + 1] = dg->par_off[b1] - dg->par_off[b0];
+                sbase[r + 1] = slots;
+                lbase[r + 1] = maxd + 1;
+                route_slot[r + 1] = 8 * (b1 - b0) + 1;
+            }
+        });
+        for (int64_t r = 0; r < n; r++) {
+            nbase[r + 1] += nbase[r]; ebase[r + 1] += ebase[r]; sbase[r + 1] += sbase[r];
+            lbase[r + 1] += lbase[r]; route_slot[r + 1] += route_slot[r];
+        }
+        const int64_t NN = nbase[n], NE = ebase[n], NS = sbase[n], NL = lbase[n];
+        std::vector<int> rnets(n), node(NN), par(NE), slot(NE), child_num(NN), loc(NN), enode(NE);
+        std::vector<int64_t> par_off(NN + 1), csum(NN + 1);
+        const bool batched = form == IGR_FORM_BATCHED;
+        std::vector<int> nlevels(batched ? n : 0), dnode(batched ? NN : 0);
+        std::vector<int64_t> lvl_node_off(batched ? NL + n : 0), lvl_edge_off(batched ? NL + n : 0),
+            dedge(batched ? NE : 0), nl_off(batched ? n : 0);
+        pfor(n, [&](int64_t a0, int64_t a1) {
+            std::vector<int64_t> cnt;
+            for (int64_t r = a0; r < a1; r++) {
+                const int64_t i = rorder[r], b0 = dg->node_off[i], b1 = dg->node_off[i + 1];
+                const int64_t g0 = nbase[r];
+                int64_t e = ebase[r], s = sbase[r];
+                rnets[r] = nets[i];
+                for (int64_t k = b0; k < b1; k++) {
+                    const int64_t gk = g0 + (k - b0);
+                    node[gk] = dg->nodes[k];
+                    child_num[gk] = dg->child_num[k];
+                    loc[gk] = (int)(k - b0);
+                    csum[gk] = s;
+                    s += dg->child_num[k];
+                    par_off[gk] = e;
+                    for (int64_t q = dg->par_off[k]; q < dg->par_off[k + 1]; q++, e++) {
+                        par[e] = (int)(g0 + dg->par[q]);
+                        slot[e] = dg->slot[q];
+                        enode[e] = (int)gk;
+                    }
+                }
+                if (!batched) continue;
+                // Depth levels of this net: nodes by depth (stable), then the
+                // parent edges of each level's nodes.
+                const int nl = (int)(lbase[r + 1] - lbase[r]);
+                const int64_t lo = lbase[r] + r;  // nl+1 boundaries per net
+                nlevels[r] = nl;
+                nl_off[r] = lo;
+                cnt.assign(nl + 1, 0);
+                for (int64_t k = b0; k < b1; k++) cnt[dg->depth[k] + 1]++;
+                for (int d = 0; d < nl; d++) cnt[d + 1] += cnt[d];
+                for (int d = 0; d <= nl; d++) lvl_node_off[lo + d] = g0 + cnt[d];
+                std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+                for (int64_t k = b0; k < b1; k++) dnode[g0 + pos[dg->depth[k]]++] = (int)(g0 + (k - b0));
+                int64_t ed = ebase[r];
+                for (int d = 0; d < nl; d++) {
+                    lvl_edge_off[lo + d] = ed;
+                    for (int64_t q = lvl_node_off[lo + d]; q < lvl_node_off[lo + d + 1]; q++) {
+                        const int64_t gk = dnode[q], k = b0 + (gk - g0);
+                        const int64_t eb = par_off[gk];
+                        for (int64_t t = 0; t < dg->par_off[k + 1] - dg->par_off[k]; t++) dedge[ed++] = eb + t;
+                    }
+                }
+                lvl_edge_off[lo + nl] = ed;
+            }
+        });
+        par_off[NN] = NE;
+        csum[NN] = NS;
+        std::vector<DagBatch> batches;
+        std::vector<int64_t> lvl_off(1, 0);
+        std::vector<int> lvl_nodes;
+        int64_t mx_nodes = 0, mx_edges = 0, mx_slots = 0;
+        for (int b = 0; b < nb; b++) {
+            DagBatch B{};
+            B.net0 = cntb[b];
+            B.net1 = cntb[b + 1];
+            B.node0 = nbase[B.net0];
+            B.node1 = nbase[B.net1];
+            B.edge0 = ebase[B.net0];
+            B.edge1 = ebase[B.net1];
+            B.slot0 = sbase[B.net0];
+            B.slot1 = sbase[B.net1];
+            int maxd = 0;
+            for (int64_t r = B.net0; r < B.net1; r++) maxd = std::max<int>(maxd, (int)(lbase[r + 1] - lbase[r]) - 1);
             B.levels = maxd + 1;
             B.lvl0 = (int64_t)lvl_off.size() - 1;
-            // Nodes of the batch bucketed by depth (stable in node order).
-            std::vector<int64_t> cnt(B.levels + 1, 0);
-            std::vector<int> dep(B.node1 - B.node0);
-            {
-                int64_t k = 0;
-                for (int64_t i : bl[b])
-                    for (int64_t q = dg->node_off[i]; q < dg->node_off[i + 1]; q++) dep[k++] = dg->depth[q];
-            }
-            for (int dd : dep) cnt[dd + 1]++;
-            for (int d = 0; d < B.levels; d++) cnt[d + 1] += cnt[d];
-            std::vector<int> bucket(dep.size());
-            {
+            if (!batched) {
+                // All nodes of the batch bucketed by depth (stable in node order).
+                std::vector<int64_t> cnt(B.levels + 1, 0);
+                for (int64_t r = B.net0; r < B.net1; r++) {
+                    const int64_t i = rorder[r];
+                    for (int64_t q = dg->node_off[i]; q < dg->node_off[i + 1]; q++) cnt[dg->depth[q] + 1]++;
+                }
+                for (int d = 0; d < B.levels; d++) cnt[d + 1] += cnt[d];
+                const int64_t base = (int64_t)lvl_nodes.size();
+                lvl_nodes.resize(base + (B.node1 - B.node0));
                 std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
-                for (size_t k = 0; k < dep.size(); k++) bucket[pos[dep[k]]++] = (int)(B.node0 + k);
+                for (int64_t r = B.net0; r < B.net1; r++) {
+                    const int64_t i = rorder[r];
+                    for (int64_t q = dg->node_off[i]; q < dg->node_off[i + 1]; q++)
+                        lvl_nodes[base + pos[dg->depth[q]]++] = (int)(nbase[r] + (q - dg->node_off[i]));
+                }
+                for (int d = 0; d < B.levels; d++) lvl_off.push_back(base + cnt[d + 1]);
             }
-            const int64_t base = (int64_t)lvl_nodes.size();
-            lvl_nodes.insert(lvl_nodes.end(), bucket.begin(), bucket.end());
-            for (int d = 0; d < B.levels; d++) lvl_off.push_back(base + cnt[d + 1]);
             mx_nodes = std::max(mx_nodes, B.node1 - B.node0);
             mx_edges = std::max(mx_edges, B.edge1 - B.edge0);
             mx_slots = std::max(mx_slots, B.slot1 - B.slot0);
             batches.push_back(B);
         }
-        // lvl_off holds, per batch, levels+1 boundaries sharing endpoints.
         D.nets.upload(rnets.data(), rnets.size(), s);
         D.node.upload(node.data(), node.size(), s);
         D.par.upload(par.data(), par.size(), s);
@@ -470,11 +538,20 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
         D.child_num.upload(child_num.data(), child_num.size(), s);
         D.loc.upload(loc.data(), loc.size(), s);
         D.enode.upload(enode.data(), enode.size(), s);
-        D.lvl_nodes.upload(lvl_nodes.data(), lvl_nodes.size(), s);
-        D.node_off.upload(node_off.data(), node_off.size(), s);
+        D.node_off.upload(nbase.data(), nbase.size(), s);
         D.par_off.upload(par_off.data(), par_off.size(), s);
         D.csum.upload(csum.data(), csum.size(), s);
         D.route_slot.upload(route_slot.data(), route_slot.size(), s);
+        if (batched) {
+            D.nl_off.upload(nl_off.data(), nl_off.size(), s);
+            D.nlevels.upload(nlevels.data(), nlevels.size(), s);
+            D.lvl_node_off.upload(lvl_node_off.data(), lvl_node_off.size(), s);
+            D.lvl_edge_off.upload(lvl_edge_off.data(), lvl_edge_off.size(), s);
+            D.dnode.upload(dnode.data(), dnode.size(), s);
+            D.dedge.upload(dedge.data(), dedge.size(), s);
+        } else {
+            D.lvl_nodes.upload(lvl_nodes.data(), lvl_nodes.size(), s);
+        }
         D.costs.alloc(mx_nodes * ML);
         D.lo.alloc(mx_nodes);
         D.emit.alloc(mx_nodes * 9);
@@ -484,8 +561,9 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
         D.best_change.alloc(std::max<int64_t>(1, mx_edges) * LL);
         NetsDev& N = c->nets;
         N.dag_routes.push_back(new DevBuf<int>());
-        N.dag_routes.back()->alloc(route_tot);
+        N.dag_routes.back()->alloc(route_slot[n]);
         int* rbase = N.dag_routes.back()->p;
+        double tp = now_s();
 
         for (const DagBatch& B : batches) {
             const int* bn = D.nets.p + B.net0;
@@ -496,18 +574,23 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
             const int s2 = ++c->stamp;
             launch_update_cost(c);
             launch_presum(c);
-            launch_dag_batch_plain(c, B, s2, lvl_off);
-            launch_dag_compact(c, B);
+            if (batched) {
+                launch_dag_batch_wave(c, B, s2);
+            } else {
+                launch_dag_batch_plain(c, B, s2, lvl_off);
+                launch_dag_compact(c, B);
+            }
             launch_wire_commit(c, s2);
             launch_via_commit(c, bn, cnt, s2, 1);
-            if (form == IGR_FORM_PLAIN) HIP_OK(hipStreamSynchronize(s));
+            if (!batched) HIP_OK(hipStreamSynchronize(s));
         }
         HIP_OK(hipStreamSynchronize(s));
-        for (size_t k = 0; k < rnets.size(); k++) {
-            N.h_rptr[rnets[k]] = rbase + route_slot[k];
-            N.h_rcap[rnets[k]] = (k + 1 < route_slot.size() ? route_slot[k + 1] : route_tot) - route_slot[k];
+        for (int64_t r = 0; r < n; r++) {
+            N.h_rptr[rnets[r]] = rbase + route_slot[r];
+            N.h_rcap[rnets[r]] = route_slot[r + 1] - route_slot[r];
         }
         double t2 = now_s();
+        c->last_dag_prep_s = tp - t1;
         if (st) *st = igr_pass_stats{nb, t1 - t0, t2 - t1, n};
     });
 }
@@ -535,6 +618,28 @@ int igr_congestion_view(igr_ctx* c, float* xsum, float* ysum) {
 }
 
 int igr_stamp(igr_ctx* c) { return c->stamp; }
+
+int igr_set_profiling(igr_ctx* c, int on) {
+    return guard([&]() {
+        HIP_OK(hipStreamSynchronize(c->stream));
+        c->prof.reset();
+        c->prof.on = on != 0;
+    });
+}
+
+int igr_kernel_times(igr_ctx* c, double* ms, int64_t* launches) {
+    static_assert(K_NCLASS == IGR_KCLASSES, "kernel class count");
+    return guard([&]() {
+        HIP_OK(hipStreamSynchronize(c->stream));
+        for (int k = 0; k < K_NCLASS; k++) { ms[k] = 0; launches[k] = 0; }
+        for (size_t i = 0; i < c->prof.rec.size(); i++) {
+            float t = 0;
+            HIP_OK(hipEventElapsedTime(&t, c->prof.rec[i].first, c->prof.rec[i].second));
+            ms[c->prof.cls[i]] += t;
+            launches[c->prof.cls[i]]++;
+        }
+    });
+}
 
 int igr_expf(const float* in, float* out, int64_t n) {
     return guard([&]() {

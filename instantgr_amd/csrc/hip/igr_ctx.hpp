@@ -86,6 +86,9 @@ struct DagDev {
     DevBuf<int> lo, best_change, emit;
     DevBuf<int> pt;
     DevBuf<int64_t> pt_off;
+    // Batched form: depth levels of every net (nodes and their parent edges).
+    DevBuf<int64_t> nl_off, lvl_node_off, lvl_edge_off, dedge;
+    DevBuf<int> nlevels, dnode;
 };
 
 struct Scratch {
@@ -98,9 +101,41 @@ struct Scratch {
     DevBuf<float> xsum, ysum;
 };
 
+// Per-kernel-class device time, measured with events on the context stream
+// when enabled (igr_set_profiling).
+enum KClass {
+    K_COST = 0, K_PRESUM, K_LSHAPE, K_WIRE_COMMIT, K_VIA_COMMIT, K_DAG_EDGES, K_DAG_DP, K_DAG_EXTRACT, K_SCHED,
+    K_TRACK_FUSED, K_RIP, K_NCLASS
+};
+
+struct KProf {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<int> cls;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> rec;
+    hipEvent_t get() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void reset() {
+        used = 0;
+        cls.clear();
+        rec.clear();
+    }
+    ~KProf() {
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
 } // namespace igr_dev
 
 struct igr_ctx {
+    igr_dev::KProf prof;
     igr_dev::GridDev g{};
     hipStream_t stream = nullptr;
     int device = 0;
@@ -114,10 +149,33 @@ struct igr_ctx {
     igr_dev::Scratch scratch;
     igr_dev::DagDev dag;
     int stamp = 0;
+    double last_dag_prep_s = 0;   // host preparation time of the last DAG pass
     int64_t cells() const { return (int64_t)g.L * g.X * g.Y; }
 };
 
 namespace igr_dev {
+// Records the device time of the launches made during its lifetime.
+struct KScope {
+    igr_ctx* c;
+    int k;
+    hipEvent_t a = nullptr;
+    KScope(igr_ctx* c_, int k_) : c(c_), k(k_) {
+        if (c->prof.on) {
+            a = c->prof.get();
+            HIP_OK(hipEventRecord(a, c->stream));
+        }
+    }
+    ~KScope() {
+        if (c->prof.on && a) {
+            hipEvent_t b = c->prof.get();
+            if (hipEventRecord(b, c->stream) == hipSuccess) {
+                c->prof.cls.push_back(k);
+                c->prof.rec.emplace_back(a, b);
+            }
+        }
+    }
+};
+
 // Kernel launchers (defined in the .hip files).
 void launch_update_cost(igr_ctx* c);
 void launch_presum(igr_ctx* c);
@@ -142,5 +200,7 @@ struct DagBatch {
 };
 void launch_dag_batch_plain(igr_ctx* c, const DagBatch& b, int stamp, const std::vector<int64_t>& h_lvl_off);
 void launch_dag_compact(igr_ctx* c, const DagBatch& b);
+// Batched form: one workgroup per net, all depth levels in one launch.
+void launch_dag_batch_wave(igr_ctx* c, const DagBatch& b, int stamp);
 void launch_expf(const float* in, float* out, int64_t n, hipStream_t s);
 } // namespace igr_dev

@@ -163,6 +163,7 @@ __global__ void __launch_bounds__(TPB) k_lshape_plain(GridDev g, const int* nets
 
 void launch_lshape_plain(igr_ctx* c, const int* d_nets, const int64_t* d_scr_off, int64_t n, int stamp) {
     if (n == 0) return;
+    KScope ks(c, K_LSHAPE);
     const unsigned grid = (unsigned)((n + TPB - 1) / TPB);
     NetsDev& N = c->nets;
 #define CASE(LV)                                                                                               \

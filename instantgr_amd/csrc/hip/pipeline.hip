@@ -1,0 +1,85 @@
+// End-to-end routing stage on one GPU: the reference's route()
+// (src/main.cpp:22-73) with the rip-up-and-reroute step repeated.
+//
+//   trees (host, already built) -> pattern pass -> for each RRR iteration:
+//   congestion view -> rip-up marking -> detour DAGs (host threads) ->
+//   DAG pass.
+//
+// The reference runs one RRR iteration and asserts on a second rip-up
+// (src/graph.hpp:446-452); here every iteration marks nets over their current
+// routes, whichever pass produced them.  'mode' follows src/main.cpp:39-40:
+// 1 above 5M subnets, 2 above 20M, set after the pattern pass.
+#include "igr_ctx.hpp"
+#include "igr_host.h"
+#include "dag.hpp"
+#include "prep.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <numeric>
+
+using namespace igr_dev;
+
+namespace {
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+} // namespace
+
+int igr_run(igr_ctx* c, igrh_design* h, const igr_run_opts* o, igr_run_stats* st) {
+    igr_run_stats S{};
+    const int form = o ? o->form : IGR_FORM_PLAIN;
+    const int iters = o ? o->rrr_iterations : 3;
+    const int threads = o && o->threads > 0 ? o->threads : 8;
+    double t0 = now_s();
+    const igr::PatternInputs& pi = *igrh_inputs_ref(h);
+    const igr::Subnets& sn = *igrh_subnets_ref(h);
+    igr_net_desc nd{pi.n, pi.node_off.data(), pi.node.data(), pi.parent.data(), pi.pt_off.data(), pi.pt.data(),
+                    pi.hs_off.data(), pi.hs.data(), pi.vs_off.data(), pi.vs.data()};
+    int rc = igr_set_nets(c, &nd);
+    if (rc) return rc;
+    double t1 = now_s();
+    S.upload_s = t1 - t0;
+    igr_pass_stats ps{};
+    rc = igr_pattern_route(c, pi.order.data(), pi.n, form, &ps);
+    if (rc) return rc;
+    double t2 = now_s();
+    S.pattern_s = t2 - t1;
+    S.pattern_batches = ps.batches;
+    S.pattern_schedule_s = ps.schedule_s;
+    const int mode = pi.n > 20000000 ? 2 : (pi.n > 5000000 ? 1 : 0);
+    const int64_t XYc = (int64_t)c->g.X * c->g.Y;
+    std::vector<float> xs(XYc), ys(XYc);
+    std::vector<uint8_t> is_of(pi.n);
+    for (int it = 0; it < iters && it < IGR_MAX_RRR; it++) {
+        double a = now_s();
+        if ((rc = igr_congestion_view(c, xs.data(), ys.data()))) return rc;
+        if ((rc = igr_ripup_mark(c, 0, is_of.data()))) return rc;
+        std::vector<int> nets;
+        for (int64_t i = 0; i < pi.n; i++)
+            if (is_of[i]) nets.push_back((int)i);
+        std::stable_sort(nets.begin(), nets.end(), [&](int x, int y) { return sn.hpwl[x] > sn.hpwl[y]; });
+        S.ripped[it] = (int64_t)nets.size();
+        if (nets.empty()) break;
+        double b = now_s();
+        if ((rc = igrh_build_dags(h, nets.data(), (int64_t)nets.size(), xs.data(), ys.data(), mode, threads))) {
+            return IGR_EINVAL;
+        }
+        double cc = now_s();
+        const igr::DagSet& D = *igrh_dags_ref(h);
+        igr_dag_desc dd{D.n, D.node_off.data(), D.node.data(), D.par_off.data(), D.par.data(), D.slot.data(),
+                        D.child_num.data(), D.depth.data(), D.pt_off.data(), D.pt.data()};
+        if ((rc = igr_dag_pass(c, nets.data(), (int64_t)nets.size(), mode, form, &dd, &ps))) return rc;
+        double e = now_s();
+        S.rrr_view_s[it] = b - a;
+        S.rrr_dag_build_s[it] = cc - b;
+        S.rrr_route_s[it] = e - cc;
+        S.rrr_batches[it] = ps.batches;
+        S.rrr_dag_nodes[it] = D.node_off.empty() ? 0 : D.node_off.back();
+        S.iterations = it + 1;
+    }
+    S.total_s = now_s() - t0;
+    S.subnets = pi.n;
+    if (st) *st = S;
+    return IGR_OK;
+}

@@ -35,6 +35,23 @@ class PassStats(C.Structure):
     _fields_ = [("batches", C.c_int), ("schedule_s", C.c_double), ("route_s", C.c_double), ("nets", C.c_int64)]
 
 
+MAX_RRR = 8
+KCLASSES = ["cost_update", "presum", "lshape_dp", "wire_commit", "via_commit", "dag_edges", "dag_dp",
+            "dag_extract", "scheduler", "track_fused", "ripup"]
+
+
+class RunOpts(C.Structure):
+    _fields_ = [("form", C.c_int), ("rrr_iterations", C.c_int), ("threads", C.c_int)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("upload_s", C.c_double), ("pattern_s", C.c_double), ("pattern_schedule_s", C.c_double),
+                ("total_s", C.c_double), ("pattern_batches", C.c_int), ("iterations", C.c_int),
+                ("subnets", C.c_int64), ("ripped", C.c_int64 * MAX_RRR), ("rrr_batches", C.c_int * MAX_RRR),
+                ("rrr_dag_nodes", C.c_int64 * MAX_RRR), ("rrr_view_s", C.c_double * MAX_RRR),
+                ("rrr_dag_build_s", C.c_double * MAX_RRR), ("rrr_route_s", C.c_double * MAX_RRR)]
+
+
 # Every entry point of include/igr.h with its ctypes signature.
 SIGNATURES = {
     "igr_create": (C.c_int, [C.POINTER(GridDesc), C.POINTER(C.c_void_p)]),
@@ -58,6 +75,9 @@ SIGNATURES = {
     "igr_ripup_mark": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "igr_congestion_view": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "igr_stamp": (C.c_int, [C.c_void_p]),
+    "igr_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "igr_kernel_times": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "igr_run": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RunOpts), C.POINTER(RunStats)]),
     "igr_expf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
 }
 
@@ -221,6 +241,28 @@ class Router:
         ys = np.empty(self.X * self.Y, np.float32)
         _check(lib().igr_congestion_view(self.h, xs.ctypes.data, ys.ctypes.data))
         return xs, ys
+
+    def set_profiling(self, on=True):
+        _check(lib().igr_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_times(self):
+        ms = (C.c_double * len(KCLASSES))()
+        ln = (C.c_int64 * len(KCLASSES))()
+        _check(lib().igr_kernel_times(self.h, ms, ln))
+        return {k: (ms[i], ln[i]) for i, k in enumerate(KCLASSES)}
+
+    def run(self, design, form=FORM_PLAIN, rrr=3, threads=8):
+        """Pattern pass plus 'rrr' rip-up-and-reroute passes on a built design."""
+        o = RunOpts(form, rrr, threads)
+        st = RunStats()
+        _check(lib().igr_run(self.h, design.h, C.byref(o), C.byref(st)))
+        self.num_nets = st.subnets
+        it = st.iterations
+        return {"upload_s": st.upload_s, "pattern_s": st.pattern_s, "pattern_schedule_s": st.pattern_schedule_s,
+                "total_s": st.total_s, "pattern_batches": st.pattern_batches, "iterations": it,
+                "subnets": st.subnets, "ripped": list(st.ripped[:it]), "rrr_batches": list(st.rrr_batches[:it]),
+                "rrr_dag_nodes": list(st.rrr_dag_nodes[:it]), "rrr_view_s": list(st.rrr_view_s[:it]),
+                "rrr_dag_build_s": list(st.rrr_dag_build_s[:it]), "rrr_route_s": list(st.rrr_route_s[:it])}
 
     @property
     def stamp(self):

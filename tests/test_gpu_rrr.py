@@ -18,14 +18,15 @@ def _routes(h):
     return [data[off[i]:off[i] + data[off[i]]] for i in range(len(off) - 1)]
 
 
-def test_rrr_three_passes_c1(design_c1):
+@pytest.mark.parametrize("form", [gpu.FORM_PLAIN, gpu.FORM_BATCHED], ids=["plain", "batched"])
+def test_rrr_three_passes_c1(design_c1, form):
     d = design_c1
     r = gpu.Router.from_design(d)
     o = orc.Oracle.from_design(d)
     r.set_nets_from_design(d)
     o.set_nets_from_design(d)
     order = d.array("order")
-    r.pattern_route(order)
+    r.pattern_route(order, form=form)
     o.pattern_route(order)
     hp = d.array("hpwl")
     for it in range(3):
@@ -37,7 +38,7 @@ def test_rrr_three_passes_c1(design_c1):
         nets = np.nonzero(ig)[0].astype(np.int32)
         nets = nets[np.argsort(-hp[nets], kind="stable")]
         dg = d.build_dags(nets, xg, yg, 0, 8)
-        st = r.dag_pass(nets, dg)
+        st = r.dag_pass(nets, dg, form=form)
         nb = orc.dag_pass(o, nets, dg)
         assert st["batches"] == nb, it
         assert np.array_equal(r.demand().view(np.uint32), o.demand().view(np.uint32)), it
