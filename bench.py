@@ -150,6 +150,9 @@ def main():
                                                    "pattern_s": st["pattern_s"],
                                                    "rrr_dag_build_s": st["rrr_dag_build_s"],
                                                    "rrr_route_s": st["rrr_route_s"],
+                                                   "rrr_schedule_s": st["rrr_schedule_s"],
+                                                   "rrr_prep_s": st["rrr_prep_s"],
+                                                   "pattern_schedule_s": st["pattern_schedule_s"],
                                                    "kernel_ms": {k: round(v[0], 1) for k, v in kt.items() if v[1]}})),
               file=sys.stderr, flush=True)
         r.close()

@@ -134,6 +134,7 @@ typedef struct igr_run_stats {
     int rrr_batches[IGR_MAX_RRR];
     int64_t rrr_dag_nodes[IGR_MAX_RRR];
     double rrr_view_s[IGR_MAX_RRR], rrr_dag_build_s[IGR_MAX_RRR], rrr_route_s[IGR_MAX_RRR];
+    double rrr_schedule_s[IGR_MAX_RRR], rrr_prep_s[IGR_MAX_RRR];
 } igr_run_stats;
 
 struct igrh_design;

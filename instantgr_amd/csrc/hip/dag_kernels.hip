@@ -64,6 +64,7 @@ __device__ __forceinline__ double wseg(const GridDev& g, int l, int xmin, int xm
 __device__ __forceinline__ void add_segment(const GridDev& g, int l, int minx, int maxx, int miny, int maxy,
                                             int stamp) {
     atomicAdd(&g.pre[gidx(g, l, minx, miny)], 1);
+    if (g.dirty) g.dirty[track_id(g, l, minx, miny)] = 1;
     if (minx == maxx) {
         atomicAdd(&g.pre[gidx(g, l, minx, maxy)], -1);
         g.ts[gidx(g, l, minx, maxy)] = stamp;
@@ -252,6 +253,7 @@ __device__ void extract_node(const DagArgs& a, int node) {
                             add_segment(g, tl, cx, cx, min(py, ch), max(py, ch), a.stamp);
                             push(fl * XY + cx * g.Y + min(cy, ch), fl * XY + cx * g.Y + max(cy, ch));
                             push(min(fl, tl) * XY + cx * g.Y + ch, max(fl, tl) * XY + cx * g.Y + ch);
+                            if (g.vc) emit_via(g, fl, tl, cx, ch);
                             push(tl * XY + cx * g.Y + min(py, ch), tl * XY + cx * g.Y + max(py, ch));
                         }
                     } else if (py == cy && cx != px) {
@@ -266,6 +268,7 @@ __device__ void extract_node(const DagArgs& a, int node) {
                             add_segment(g, tl, min(px, ch), max(px, ch), cy, cy, a.stamp);
                             push(fl * XY + min(cx, ch) * g.Y + cy, fl * XY + max(cx, ch) * g.Y + cy);
                             push(min(fl, tl) * XY + ch * g.Y + cy, max(fl, tl) * XY + ch * g.Y + cy);
+                            if (g.vc) emit_via(g, fl, tl, ch, cy);
                             push(tl * XY + min(px, ch) * g.Y + cy, tl * XY + max(px, ch) * g.Y + cy);
                         }
                     }
@@ -304,7 +307,10 @@ __device__ void extract_node(const DagArgs& a, int node) {
         minl = min(lc, minl);
         maxl = max(lc, maxl);
     }
-    if (minl < maxl) push((minl - 1) * XY + cx * g.Y + cy, (maxl - 1) * XY + cx * g.Y + cy);
+    if (minl < maxl) {
+        push((minl - 1) * XY + cx * g.Y + cy, (maxl - 1) * XY + cx * g.Y + cy);
+        if (g.vc) emit_via(g, minl - 1, maxl - 1, cx, cy);
+    }
     em[0] = ne;
 }
 

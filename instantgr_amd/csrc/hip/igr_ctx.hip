@@ -71,6 +71,9 @@ int igr_create(const igr_grid_desc* d, igr_ctx** out) {
         g.w = c->w.p; g.xlen = c->xlen.p; g.ylen = c->ylen.p; g.cap = c->cap.p;
         g.demand = c->demand.p; g.wcost = c->wcost.p; g.vcost = c->vcost.p; g.presum = c->presum.p;
         g.pre = c->pre.p; g.ts = c->ts.p;
+        g.tbase[0] = 0;
+        for (int l = 0; l < g.L; l++) g.tbase[l + 1] = g.tbase[l] + (layer_dir(l, g.DIR) ? g.X : g.Y);
+        c->track_stats.zero(2, c->stream);
         launch_from_ref(c, tmp.p, c->cap.p, 4);
         HIP_OK(hipStreamSynchronize(c->stream));
     });
@@ -308,8 +311,24 @@ int igr_pattern_route(igr_ctx* c, const int* order, int64_t n, int form, igr_pas
         ensure_scratch(c, max_nodes);
         c->scratch.net_list.upload(list.data(), n, c->stream);
         c->scratch.scr_off.upload(off.data(), n, c->stream);
-        for (int b = 0; b < nb; b++)
-            plain_batch(c, c->scratch.net_list.p + cnt[b], c->scratch.scr_off.p + cnt[b], cnt[b + 1] - cnt[b]);
+        if (form == IGR_FORM_PLAIN) {
+            for (int b = 0; b < nb; b++)
+                plain_batch(c, c->scratch.net_list.p + cnt[b], c->scratch.scr_off.p + cnt[b], cnt[b + 1] - cnt[b]);
+        } else {
+            // Costs of every track once, then per batch: DP (scattering wires,
+            // vias and dirty flags), then the fused commit + cost pass over the
+            // touched tracks.  The last pass leaves the costs as the plain
+            // form does (computed before the last batch).
+            BatchedState bs(c);
+            launch_track_fused(c, 0, 0, true);
+            for (int b = 0; b < nb; b++) {
+                const int stamp = ++c->stamp;
+                launch_lshape_plain(c, c->scratch.net_list.p + cnt[b], c->scratch.scr_off.p + cnt[b],
+                                    cnt[b + 1] - cnt[b], stamp);
+                launch_track_fused(c, stamp, 0, b + 1 < nb);
+            }
+            HIP_OK(hipStreamSynchronize(c->stream));
+        }
         HIP_OK(hipStreamSynchronize(c->stream));
         double t2 = now_s();
         if (st) {
@@ -565,24 +584,44 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
         int* rbase = N.dag_routes.back()->p;
         double tp = now_s();
 
-        for (const DagBatch& B : batches) {
-            const int* bn = D.nets.p + B.net0;
-            const int64_t cnt = B.net1 - B.net0;
-            const int s1 = ++c->stamp;
-            launch_rip_wire(c, bn, cnt, s1);
-            launch_via_commit(c, bn, cnt, s1, -1);
-            const int s2 = ++c->stamp;
-            launch_update_cost(c);
-            launch_presum(c);
-            if (batched) {
-                launch_dag_batch_wave(c, B, s2);
-            } else {
+        if (!batched) {
+            for (const DagBatch& B : batches) {
+                const int* bn = D.nets.p + B.net0;
+                const int64_t cnt = B.net1 - B.net0;
+                const int s1 = ++c->stamp;
+                launch_rip_wire(c, bn, cnt, s1);
+                launch_via_commit(c, bn, cnt, s1, -1);
+                const int s2 = ++c->stamp;
+                launch_update_cost(c);
+                launch_presum(c);
                 launch_dag_batch_plain(c, B, s2, lvl_off);
                 launch_dag_compact(c, B);
+                launch_wire_commit(c, s2);
+                launch_via_commit(c, bn, cnt, s2, 1);
+                HIP_OK(hipStreamSynchronize(s));
             }
-            launch_wire_commit(c, s2);
-            launch_via_commit(c, bn, cnt, s2, 1);
-            if (!batched) HIP_OK(hipStreamSynchronize(s));
+        } else {
+            // Batch b: DAG routing (scattering its wires, vias and dirty
+            // flags), the rip-up scatter of batch b+1 from its current routes,
+            // then one fused pass: commit b, rip up b+1, costs.  Stamps follow
+            // the plain sequence (rip-up, then commit, per batch).
+            BatchedState bs(c);
+            int s1 = ++c->stamp;
+            launch_rip_scatter(c, D.nets.p + batches[0].net0, batches[0].net1 - batches[0].net0);
+            launch_track_fused(c, 0, s1, true);
+            for (size_t b = 0; b < batches.size(); b++) {
+                const int s2 = ++c->stamp;
+                launch_dag_batch_wave(c, batches[b], s2);
+                if (b + 1 < batches.size()) {
+                    const DagBatch& Bn = batches[b + 1];
+                    s1 = ++c->stamp;
+                    launch_rip_scatter(c, D.nets.p + Bn.net0, Bn.net1 - Bn.net0);
+                    launch_track_fused(c, s2, s1, true);
+                } else {
+                    launch_track_fused(c, s2, 0, false);
+                }
+            }
+            bs.check();
         }
         HIP_OK(hipStreamSynchronize(s));
         for (int64_t r = 0; r < n; r++) {

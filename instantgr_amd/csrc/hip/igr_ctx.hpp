@@ -144,6 +144,10 @@ struct igr_ctx {
     igr_dev::DevBuf<float> cap, demand, wcost, vcost;
     igr_dev::DevBuf<double> presum;
     igr_dev::DevBuf<int> pre, ts;
+    // Batched form: via-cut counts, rip-up differences, dirty-track flags, the
+    // bad-route flag and the fused-pass cell counters (processed, ripped).
+    igr_dev::DevBuf<int> vc, rpre, rcov, rvc, dirty, bad;
+    igr_dev::DevBuf<unsigned long long> track_stats;
     igr_dev::NetsDev nets;
     igr_dev::Sched sched;
     igr_dev::Scratch scratch;
@@ -203,4 +207,20 @@ void launch_dag_compact(igr_ctx* c, const DagBatch& b);
 // Batched form: one workgroup per net, all depth levels in one launch.
 void launch_dag_batch_wave(igr_ctx* c, const DagBatch& b, int stamp);
 void launch_expf(const float* in, float* out, int64_t n, hipStream_t s);
+
+// Batched-form grid state for the lifetime of one pass: clears the scatter
+// arrays, flags every track dirty and points the grid at them; the plain
+// kernels see null pointers outside it.
+struct BatchedState {
+    igr_ctx* c;
+    explicit BatchedState(igr_ctx* c_);
+    ~BatchedState();
+    BatchedState(const BatchedState&) = delete;
+    BatchedState& operator=(const BatchedState&) = delete;
+    void check();   // throws if a ripped route could not be scattered
+};
+// Fused per-track pass: commit of the routed batch (s_commit > 0), rip-up of
+// the next one (s_rip > 0), then costs and prefix sums when 'cost' is set.
+void launch_track_fused(igr_ctx* c, int s_commit, int s_rip, bool cost);
+void launch_rip_scatter(igr_ctx* c, const int* d_nets, int64_t n);
 } // namespace igr_dev

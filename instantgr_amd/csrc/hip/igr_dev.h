@@ -32,6 +32,16 @@ struct GridDev {
     double* presum;
     int* pre;                     // wire demand differences
     int* ts;                      // commit timestamps
+    // Batched form only (null in the plain form): per-cell via-cut counts of
+    // the routed batch, rip-up differences of the next batch, and a dirty flag
+    // per track (tracks ordered by layer, then x or y; tbase[l] = first track
+    // of layer l).
+    int* vc;
+    int* rpre;
+    int* rcov;
+    int* rvc;
+    int* dirty;
+    int tbase[MAXL + 1];
 };
 
 __host__ __device__ __forceinline__ int layer_dir(int l, int DIR) { return (l & 1) ^ DIR; }
@@ -39,6 +49,22 @@ __host__ __device__ __forceinline__ int layer_dir(int l, int DIR) { return (l & 
 // Device offset of cell (l, x, y).
 __host__ __device__ __forceinline__ int64_t gidx(const GridDev& g, int l, int x, int y) {
     return (int64_t)l * g.XYc + (layer_dir(l, g.DIR) ? (int64_t)x * g.Y + y : (int64_t)y * g.X + x);
+}
+
+// Track holding cell (l, x, y).
+__device__ __forceinline__ int track_id(const GridDev& g, int l, int x, int y) {
+    return g.tbase[l] + (layer_dir(l, g.DIR) ? x : y);
+}
+
+// Batched form: record the via cuts of a via between layers l0 and l1 at
+// (x, y) for the fused track update (each cut l in [min, max) adds demand on
+// layer l, src/graph.hpp:511-541).
+__device__ __forceinline__ void emit_via(const GridDev& g, int l0, int l1, int x, int y) {
+    const int lo = l0 < l1 ? l0 : l1, hi = l0 < l1 ? l1 : l0;
+    for (int l = lo; l < hi; l++) {
+        atomicAdd(&g.vc[gidx(g, l, x, y)], 1);
+        g.dirty[track_id(g, l, x, y)] = 1;
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -24,6 +24,7 @@ __device__ __forceinline__ double wseg(const GridDev& g, int l, int xmin, int xm
 __device__ __forceinline__ void add_segment(const GridDev& g, int l, int minx, int maxx, int miny, int maxy,
                                             int stamp) {
     atomicAdd(&g.pre[gidx(g, l, minx, miny)], 1);
+    if (g.dirty) g.dirty[track_id(g, l, minx, miny)] = 1;
     if (minx == maxx) {
         atomicAdd(&g.pre[gidx(g, l, minx, maxy)], -1);
         g.ts[gidx(g, l, minx, maxy)] = stamp;
@@ -132,7 +133,10 @@ __global__ void __launch_bounds__(TPB) k_lshape_plain(GridDev g, const int* nets
         if (i > 0) lr[i] = prev[lr[par[i]]] / LL;
         int minl = lr[i] / L, maxl = lr[i] % L;
         int x = nd_pos[i] / Y % X, y = nd_pos[i] % Y;
-        if (minl < maxl) push(minl * XY + x * Y + y, maxl * XY + x * Y + y);
+        if (minl < maxl) {
+            push(minl * XY + x * Y + y, maxl * XY + x * Y + y);
+            if (g.vc) emit_via(g, minl, maxl, x, y);
+        }
         if (i == 0) continue;
         int p = par[i];
         int px = nd_pos[p] / Y % X, py = nd_pos[p] % Y;
@@ -146,12 +150,14 @@ __global__ void __launch_bounds__(TPB) k_lshape_plain(GridDev g, const int* nets
             push(curl * XY + x * Y + y, curl * XY + x * Y + py);
             add_segment(g, curl, x, x, miny, maxy, stamp);
             push(curl * XY + x * Y + py, parl * XY + x * Y + py);
+            if (g.vc) emit_via(g, curl, parl, x, py);
             push(parl * XY + x * Y + py, parl * XY + px * Y + py);
             add_segment(g, parl, minx, maxx, py, py, stamp);
         } else {
             push(curl * XY + x * Y + y, curl * XY + px * Y + y);
             add_segment(g, curl, minx, maxx, y, y, stamp);
             push(curl * XY + px * Y + y, parl * XY + px * Y + y);
+            if (g.vc) emit_via(g, curl, parl, px, y);
             push(parl * XY + px * Y + y, parl * XY + px * Y + py);
             add_segment(g, parl, px, px, miny, maxy, stamp);
         }

@@ -74,6 +74,8 @@ int igr_run(igr_ctx* c, igrh_design* h, const igr_run_opts* o, igr_run_stats* st
         S.rrr_view_s[it] = b - a;
         S.rrr_dag_build_s[it] = cc - b;
         S.rrr_route_s[it] = e - cc;
+        S.rrr_schedule_s[it] = ps.schedule_s;
+        S.rrr_prep_s[it] = c->last_dag_prep_s;
         S.rrr_batches[it] = ps.batches;
         S.rrr_dag_nodes[it] = D.node_off.empty() ? 0 : D.node_off.back();
         S.iterations = it + 1;

@@ -18,13 +18,15 @@
 //     init_high - 1 (src/database_cuda.hpp:1747), is kept as written.
 // Nodes that are neither pins nor tree Steiner points carry layer L, like the
 // reference's MAX_LAYER-1 with its 9 routing layers.
+//
+// Each worker thread reuses one builder (no per-net allocations once warm)
+// and writes a chunk of nets into its own buffer; the chunks are then copied
+// into the final arrays in parallel.
 #include "dag.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
-#include <functional>
-#include <queue>
-#include <stack>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -33,53 +35,59 @@
 namespace igr {
 namespace {
 
-// src/database_cuda.hpp:826-853: breadth-first from every leaf; the answer is
-// the node of the last queue entry popped.
-int select_root(const std::vector<std::vector<int>>& adj) {
-    const int n = (int)adj.size();
-    std::queue<std::pair<int, int>> q;
-    std::vector<char> visited(n, 0);
-    int select = -1;
-    for (int i = 0; i < n; i++)
-        if (adj[i].size() == 1) q.push({i, -1});
-    while (!q.empty()) {
-        auto f = q.front();
-        q.pop();
-        select = f.first;
-        if (visited[select]) continue;
-        visited[select] = 1;
-        for (int j : adj[select])
-            if (j != f.second) q.push({j, select});
-    }
-    return select < 0 ? 0 : select;
-}
-
 struct Builder {
     const Grid& g;
     const float* xsum;
     const float* ysum;
     int mode;
     int X, Y, XYc, L;
-    // tree
-    std::vector<std::vector<int>> adj;
-    std::vector<int> pos;              // encoded with layer (L = Steiner)
+    float ratio = 0.1f;
+    int num_tracks = 6;
+    // tree (CSR borrowed from the subnets)
+    int tn = 0;
+    const int* pos = nullptr;
+    const int64_t* aoff = nullptr;   // relative offsets, tn+1
+    const int* tadj = nullptr;
     int root = 0;
     // congestion regions
     std::vector<int> region[2];
     std::vector<std::pair<int, int>> ranges[2];
     std::vector<std::vector<int>> stems[2];
-    float ratio = 0.1f;
-    int num_tracks = 6;
     // DAG under construction
-    std::vector<int> nodes;
-    std::vector<std::vector<int>> parents;  // parent*10 + child slot
-    std::vector<int> child_num_pre;
+    std::vector<int> nodes, child_num_pre;
+    std::vector<std::pair<int, int>> edges;   // (child, parent*10 + slot) in creation order
     std::vector<int> points;
+    // per recursion depth buffers: mirrors created at a node and their places
+    std::vector<std::vector<int>> mir_buf[2], place_buf[2];
+    // scratch
+    std::vector<std::pair<float, int>> road;
+    std::vector<int> q, visited, pcnt, poff, plist, visit, seq, st;
 
     Builder(const Grid& g_, const float* xs, const float* ys, int mode_)
         : g(g_), xsum(xs), ysum(ys), mode(mode_), X(g_.X), Y(g_.Y), XYc(g_.X * g_.Y), L(g_.L) {
         if (mode == 0 || mode == 1) { ratio = 0.1f; num_tracks = 6; }
         else { ratio = 0.15f; num_tracks = 9; }
+    }
+
+    int deg(int x) const { return (int)(aoff[x + 1] - aoff[x]); }
+
+    // src/database_cuda.hpp:826-853: breadth-first from every leaf; the answer
+    // is the node of the last queue entry popped.
+    int select_root() {
+        q.clear();
+        visited.assign(tn, 0);
+        for (int i = 0; i < tn; i++)
+            if (deg(i) == 1) { q.push_back(i); q.push_back(-1); }
+        int select = -1;
+        for (size_t h = 0; h < q.size(); h += 2) {
+            int node = q[h], fa = q[h + 1];
+            select = node;
+            if (visited[node]) continue;
+            visited[node] = 1;
+            for (int64_t a = aoff[node]; a < aoff[node + 1]; a++)
+                if (tadj[a] != fa) { q.push_back(tadj[a]); q.push_back(node); }
+        }
+        return select < 0 ? 0 : select;
     }
 
     int getRegion(int x, int dir) {
@@ -93,23 +101,25 @@ struct Builder {
         return region[dir][x];
     }
 
-    float ysumd(int x0, int y0, int x1, int y1) const {  // ysum[max] - ysum[min] pattern
+    float ysumd(int x0, int y0, int x1, int y1) const {
         return ysum[(size_t)std::max(x0, x1) * Y + std::max(y0, y1)] - ysum[(size_t)std::min(x0, x1) * Y + std::min(y0, y1)];
     }
 
     void markCongestion(int x, int par) {
-        int cur = pos[x];
-        int curx = cur / Y % X, cury = cur % Y;
-        for (int e : adj[x]) {
+        int curx = pos[x] / Y % X, cury = pos[x] % Y;
+        for (int64_t a = aoff[x]; a < aoff[x + 1]; a++) {
+            int e = tadj[a];
             if (e == par) continue;
             int ex = pos[e] / Y % X, ey = pos[e] % Y;
             int dir = -1, congestion = -1;
             if (ex == curx) {
-                bool cy = (ysum[(size_t)curx * Y + std::max(ey, cury)] - ysum[(size_t)curx * Y + std::min(ey, cury)]) > 0;
-                if (cy) { dir = 1; congestion = 1; }
+                if ((ysum[(size_t)curx * Y + std::max(ey, cury)] - ysum[(size_t)curx * Y + std::min(ey, cury)]) > 0) {
+                    dir = 1; congestion = 1;
+                }
             } else if (ey == cury) {
-                bool cx = (xsum[(size_t)std::max(ex, curx) * Y + cury] - xsum[(size_t)std::min(ex, curx) * Y + cury]) > 0;
-                if (cx) { dir = 0; congestion = 1; }
+                if ((xsum[(size_t)std::max(ex, curx) * Y + cury] - xsum[(size_t)std::min(ex, curx) * Y + cury]) > 0) {
+                    dir = 0; congestion = 1;
+                }
             } else {
                 bool cy = (ysum[(size_t)curx * Y + std::max(ey, cury)] - ysum[(size_t)curx * Y + std::min(ey, cury)]) > 0 ||
                           (ysum[(size_t)ex * Y + std::max(ey, cury)] - ysum[(size_t)ex * Y + std::min(ey, cury)]) > 0;
@@ -130,8 +140,11 @@ struct Builder {
                     target = getRegion(e, dir);
                 }
                 region[dir][e] = target;
-                if (x != root)
-                    for (int p : std::vector<int>(stems[dir][e])) stems[dir][target].push_back(p);
+                if (x != root && target != e) {
+                    auto& dst = stems[dir][target];
+                    const auto& src = stems[dir][e];
+                    dst.insert(dst.end(), src.begin(), src.end());
+                }
                 ranges[dir][target].first = std::min(ranges[dir][target].first, ranges[dir][e].first);
                 ranges[dir][target].second = std::max(ranges[dir][target].second, ranges[dir][e].second);
             } else if (congestion == 2) {
@@ -142,7 +155,8 @@ struct Builder {
         }
         for (int dir = 0; dir < 2; dir++) {
             int xr = getRegion(x, dir);
-            for (int e : adj[x]) {
+            for (int64_t a = aoff[x]; a < aoff[x + 1]; a++) {
+                int e = tadj[a];
                 if (e == par) continue;
                 int ex = pos[e] / Y % X, ey = pos[e] % Y;
                 int er = getRegion(e, dir);
@@ -165,7 +179,6 @@ struct Builder {
     int create_node(int l, int x, int y) {
         int idx = (int)nodes.size();
         nodes.push_back(l * XYc + x * Y + y);
-        parents.emplace_back();
         child_num_pre.push_back(0);
         return idx;
     }
@@ -173,10 +186,9 @@ struct Builder {
     void connect_pre(int par_idx, int cur_idx, int child_id) {
         // Edges carry parent*10 + slot (src/database_cuda.hpp:1655).
         if (child_id < 0 || child_id >= 10) throw std::runtime_error("detour DAG node with more than 10 child slots");
-        int c = nodes[cur_idx] % XYc, p = nodes[par_idx] % XYc;
-        points.push_back(c);
-        points.push_back(p);
-        parents[cur_idx].push_back(par_idx * 10 + child_id);
+        points.push_back(nodes[cur_idx] % XYc);
+        points.push_back(nodes[par_idx] % XYc);
+        edges.emplace_back(cur_idx, par_idx * 10 + child_id);
         child_num_pre[par_idx]++;
     }
 
@@ -190,35 +202,33 @@ struct Builder {
             return;
         }
         if (dist < 150) {
-            int mx = (par_x + cur_x) / 2, my = (par_y + cur_y) / 2;
-            int mid = create_node(L, mx, my);
+            int mid = create_node(L, (par_x + cur_x) / 2, (par_y + cur_y) / 2);
             connect_pre(par_idx, mid, child_id);
             connect_pre(mid, cur_idx, 0);
             return;
         }
+        int m1, m2;
         if (par_y == cur_y) {
-            int m1 = create_node(L, par_x * 2 / 3 + cur_x / 3, par_y);
-            int m2 = create_node(L, par_x * 1 / 3 + cur_x * 2 / 3, par_y);
-            connect_pre(par_idx, m1, child_id);
-            connect_pre(m1, m2, 0);
-            connect_pre(m2, cur_idx, 0);
+            m1 = create_node(L, par_x * 2 / 3 + cur_x / 3, par_y);
+            m2 = create_node(L, par_x * 1 / 3 + cur_x * 2 / 3, par_y);
         } else {
-            int m1 = create_node(L, par_x, par_y * 2 / 3 + cur_y / 3);
-            int m2 = create_node(L, par_x, par_y * 1 / 3 + cur_y * 2 / 3);
-            connect_pre(par_idx, m1, child_id);
-            connect_pre(m1, m2, 0);
-            connect_pre(m2, cur_idx, 0);
+            m1 = create_node(L, par_x, par_y * 2 / 3 + cur_y / 3);
+            m2 = create_node(L, par_x, par_y * 1 / 3 + cur_y * 2 / 3);
         }
+        connect_pre(par_idx, m1, child_id);
+        connect_pre(m1, m2, 0);
+        connect_pre(m2, cur_idx, 0);
     }
 
-    int calc_displace(int q, int dir, int reg) const {
+    int calc_displace(int qpos, int dir, int reg) const {
         int ans = 0;
-        for (int p : stems[dir][reg]) ans += std::abs(p - q);
+        for (int p : stems[dir][reg]) ans += std::abs(p - qpos);
         return ans;
     }
 
     // src/database_cuda.hpp:1733-1771.
-    std::vector<int> mirror_places(int node, int dir) {
+    void mirror_places(int node, int dir, std::vector<int>& shifts) {
+        shifts.clear();
         int reg = region[dir][node];
         int curx = pos[node] / Y % X, cury = pos[node] % Y;
         int trunk_len = ranges[dir][reg].second - ranges[dir][reg].first;
@@ -233,45 +243,57 @@ struct Builder {
         while ((orig - lo) / (step + 1) + (hi - orig) / (step + 1) >= num_tracks) step++;
         lo = orig - (orig - lo) / step * step;
         hi = orig + (hi - orig) / step * step;
-        std::vector<int> shifts;
         for (double p = lo; p <= hi; p += step) {
             int shift = (int)(p - orig);
             if (shift == 0) continue;
             shifts.push_back((int)p);
         }
-        return shifts;
     }
 
-    void dfs(int x, int par, int par_node, int child_idx, int depth, std::vector<std::vector<int>> mirrors,
-             const std::vector<std::vector<int>>& old_places) {
-        if (mirrors.empty()) mirrors.resize(2);
+    void ensure_depth(int depth) {
+        for (int d = 0; d < 2; d++)
+            while ((int)mir_buf[d].size() <= depth) {
+                mir_buf[d].emplace_back();
+                place_buf[d].emplace_back();
+            }
+    }
+
+    // mirrors / old places of the parent are mir_buf[*][depth-1] and
+    // place_buf[*][depth-1] (absent for the root).
+    void dfs(int x, int par, int par_node, int child_idx, int depth, int level) {
+        ensure_depth(level);
         int cur = pos[x];
         int curl = cur / XYc, curx = cur / Y % X, cury = cur % Y;
         int node_idx = create_node(curl, curx, cury);
-        std::vector<std::vector<int>> new_mirrors(2), places(2);
-        if (old_places.size() == 2) {
+        std::vector<int>* new_mirrors[2] = {&mir_buf[0][level], &mir_buf[1][level]};
+        std::vector<int>* places[2] = {&place_buf[0][level], &place_buf[1][level]};
+        for (int d = 0; d < 2; d++) { new_mirrors[d]->clear(); places[d]->clear(); }
+        const bool has_parent_data = level > 0;
+        if (has_parent_data) {
             for (int dir = 0; dir < 2; dir++) {
+                const std::vector<int>& mirrors = mir_buf[dir][level - 1];
+                const std::vector<int>& old_places = place_buf[dir][level - 1];
                 int rid = getRegion(region[dir][x], dir);
                 if (rid < 0) continue;
                 if (getRegion(region[dir][par], dir) == par) depth += 2;
-                if (x == rid) places[dir] = mirror_places(x, dir);
-                else places[dir] = old_places[dir];
-                if (rid != x && places[dir].size() != mirrors[dir].size())
+                if (x == rid) mirror_places(x, dir, *places[dir]);
+                else *places[dir] = old_places;
+                if (rid != x && places[dir]->size() != mirrors.size())
                     throw std::runtime_error("detour construction: mirror count mismatch");
-                for (size_t m = 0; m < places[dir].size(); m++) {
-                    int nx = dir ? places[dir][m] : curx;
-                    int ny = dir ? cury : places[dir][m];
+                for (size_t m = 0; m < places[dir]->size(); m++) {
+                    int pl = (*places[dir])[m];
+                    int nx = dir ? pl : curx;
+                    int ny = dir ? cury : pl;
                     int nm;
                     if (rid != x) {
                         nm = create_node(L, nx, ny);
-                        connect(mirrors[dir][m], nm, child_idx);
+                        connect(mirrors[m], nm, child_idx);
                     } else if (x != root) {
                         int px = nodes[par_node] / Y % X, py = nodes[par_node] % Y;
                         if (nx != px && ny != py) {
                             int t[2];
                             for (int pi = 0; pi <= 1; pi++) {
-                                int mx = pi ? nx : px, my = pi ? py : ny;
-                                t[pi] = create_node(L, mx, my);
+                                t[pi] = create_node(L, pi ? nx : px, pi ? py : ny);
                                 connect(par_node, t[pi], child_idx);
                             }
                             nm = create_node(L, nx, ny);
@@ -283,7 +305,7 @@ struct Builder {
                     } else {
                         nm = create_node(L, nx, ny);
                     }
-                    new_mirrors[dir].push_back(nm);
+                    new_mirrors[dir]->push_back(nm);
                 }
             }
         }
@@ -291,8 +313,7 @@ struct Builder {
             int px = nodes[par_node] / Y % X, py = nodes[par_node] % Y;
             if (px != curx && py != cury) {
                 for (int pi = 0; pi <= 1; pi++) {
-                    int mx = pi ? curx : px, my = pi ? py : cury;
-                    int pre = create_node(L, mx, my);
+                    int pre = create_node(L, pi ? curx : px, pi ? py : cury);
                     connect(par_node, pre, child_idx);
                     connect(pre, node_idx, 0);
                 }
@@ -314,7 +335,7 @@ struct Builder {
                         if (x2 == px && y2 == py) return false;
                         return true;
                     };
-                    std::vector<std::pair<float, int>> road;
+                    road.clear();
                     for (int d = 1; d < mz; d++) {
                         int x1, y1, x2, y2;
                         if (!mids(d, x1, y1, x2, y2)) continue;
@@ -341,12 +362,12 @@ struct Builder {
                 int rid = getRegion(region[dir][x], dir);
                 int prid = getRegion(region[dir][par], dir);
                 if (prid >= 0 && rid != prid) {
-                    for (int pm : mirrors[dir]) {
+                    const std::vector<int>& mirrors = mir_buf[dir][level - 1];
+                    for (int pm : mirrors) {
                         int qx = nodes[pm] / Y % X, qy = nodes[pm] % Y;
                         if (qx != curx && qy != cury) {
                             for (int pi = 0; pi <= 1; pi++) {
-                                int mx = pi ? curx : qx, my = pi ? qy : cury;
-                                int ins = create_node(L, mx, my);
+                                int ins = create_node(L, pi ? curx : qx, pi ? qy : cury);
                                 connect(pm, ins, child_idx);
                                 connect(ins, node_idx, 0);
                             }
@@ -363,27 +384,33 @@ struct Builder {
             if (rid < 0) continue;
             bool is_pin = curl < L;
             if (x != root && is_pin)
-                for (int m : new_mirrors[dir]) {
+                for (int m : *new_mirrors[dir]) {
                     int dup = create_node(curl, curx, cury);
-                    connect(m, dup, (int)adj[x].size() - 1);
+                    connect(m, dup, deg(x) - 1);
                 }
         }
         int idx = 0;
-        for (int e : adj[x]) {
+        for (int64_t a = aoff[x]; a < aoff[x + 1]; a++) {
+            int e = tadj[a];
             if (e == par) continue;
-            dfs(e, x, node_idx, idx++, depth + 1, new_mirrors, places);
+            dfs(e, x, node_idx, idx++, depth + 1, level + 1);
         }
     }
 
-    void build(const int* tnode, int tn, const int64_t* adj_off, const int* tadj, DagNet& out) {
-        adj.assign(tn, {});
-        pos.assign(tnode, tnode + tn);
-        for (int i = 0; i < tn; i++) adj[i].assign(tadj + adj_off[i], tadj + adj_off[i + 1]);
-        root = select_root(adj);
+    // Build one net's DAG and append it to the chunk output.
+    void build(const int* tnode, int tn_, const int64_t* adj_off_abs, const int* tadj_, DagChunk& out) {
+        tn = tn_;
+        pos = tnode;
+        tadj = tadj_;
+        rel.assign(adj_off_abs, adj_off_abs + tn + 1);
+        for (auto& v : rel) v -= adj_off_abs[0];
+        aoff = rel.data();
+        root = select_root();
         for (int d = 0; d < 2; d++) {
             region[d].assign(tn, -1);
-            ranges[d].assign(tn, {0, 0});
-            stems[d].assign(tn, {});
+            ranges[d].resize(tn);
+            if ((int)stems[d].size() < tn) stems[d].resize(tn);
+            for (int x = 0; x < tn; x++) stems[d][x].clear();
         }
         for (int x = 0; x < tn; x++) {
             int curl = pos[x] / XYc, curx = pos[x] / Y % X, cury = pos[x] % Y;
@@ -395,52 +422,84 @@ struct Builder {
         }
         markCongestion(root, -1);
         nodes.clear();
-        parents.clear();
         child_num_pre.clear();
+        edges.clear();
         points.clear();
-        dfs(root, -1, -1, -1, 0, {}, {});
+        dfs(root, -1, -1, -1, 0, 0);
         std::sort(points.begin(), points.end());
         points.erase(std::unique(points.begin(), points.end()), points.end());
 
-        // Topological renumbering (src/database_cuda.hpp:2016-2080).
+        // Parent lists in creation order (CSR by child).
         const int n = (int)nodes.size();
-        std::vector<int> visit(n, 0), seq(n, 0);
-        std::vector<int> st;
-        st.reserve(n);
-        std::function<void(int)> determine = [&](int idx) {
-            st.push_back(idx);
-            for (int content : parents[idx]) {
-                int p = content / 10;
-                if (++visit[p] == child_num_pre[p]) determine(p);
+        poff.assign(n + 1, 0);
+        for (auto& e : edges) poff[e.first + 1]++;
+        for (int i = 0; i < n; i++) poff[i + 1] += poff[i];
+        plist.resize(edges.size());
+        pcnt.assign(n, 0);
+        for (auto& e : edges) plist[poff[e.first] + pcnt[e.first]++] = e.second;
+
+        // Topological renumbering (src/database_cuda.hpp:2016-2080): a node is
+        // pushed once all its child edges are visited; popping the stack gives
+        // parents before children.
+        visit.assign(n, 0);
+        st.clear();
+        std::vector<int>& work = q;
+        for (int i = 0; i < n; i++) {
+            if (child_num_pre[i] != 0) continue;
+            // iterative form of the recursive determinSequence
+            work.clear();
+            work.push_back(i);
+            work.push_back(0);
+            st.push_back(i);
+            while (!work.empty()) {
+                int idx = work[work.size() - 2];
+                int& k = work.back();
+                if (k == poff[idx + 1] - poff[idx]) {
+                    work.pop_back();
+                    work.pop_back();
+                    continue;
+                }
+                int p = plist[poff[idx] + k] / 10;
+                k++;
+                if (++visit[p] == child_num_pre[p]) {
+                    st.push_back(p);
+                    work.push_back(p);
+                    work.push_back(0);
+                }
             }
-        };
-        for (int i = 0; i < n; i++)
-            if (child_num_pre[i] == 0) determine(i);
-        out.node.assign(n, 0);
-        out.par_off.assign(n + 1, 0);
-        out.child_num.assign(n, 0);
-        out.depth.assign(n, 0);
-        out.par.clear();
-        out.slot.clear();
+        }
+        const int64_t base_par = (int64_t)out.par.size();
+        seq.assign(n, 0);
+        const size_t nbase = out.node.size();
+        out.node.resize(nbase + n);
+        out.child_num.resize(nbase + n, 0);
+        out.depth.resize(nbase + n, 0);
+        out.npar.resize(nbase + n, 0);
         int copy = 0;
-        while (!st.empty()) {
-            int idx = st.back();
-            st.pop_back();
-            for (int content : parents[idx]) {
-                int p = content / 10, ci = content % 10;
+        for (auto it = st.rbegin(); it != st.rend(); ++it) {
+            int idx = *it;
+            int dep = 0;
+            for (int k = poff[idx]; k < poff[idx + 1]; k++) {
+                int p = plist[k] / 10, ci = plist[k] % 10;
                 int sp = seq[p];
-                out.child_num[sp] = std::max(out.child_num[sp], ci + 1);
+                out.child_num[nbase + sp] = std::max(out.child_num[nbase + sp], ci + 1);
                 out.par.push_back(sp);
                 out.slot.push_back(ci);
-                out.depth[copy] = std::max(out.depth[copy], out.depth[sp] + 1);
+                dep = std::max(dep, out.depth[nbase + sp] + 1);
             }
+            out.depth[nbase + copy] = dep;
+            out.npar[nbase + copy] = poff[idx + 1] - poff[idx];
             seq[idx] = copy;
-            out.par_off[copy + 1] = out.par_off[copy] + (int)parents[idx].size();
-            out.node[copy++] = nodes[idx];
+            out.node[nbase + copy++] = nodes[idx];
         }
         if (copy != n) throw std::runtime_error("detour DAG has a cycle");
-        out.points = points;
+        (void)base_par;
+        out.nodes_per_net.push_back(n);
+        out.points.insert(out.points.end(), points.begin(), points.end());
+        out.points_per_net.push_back((int)points.size());
     }
+
+    std::vector<int64_t> rel;
 };
 
 } // namespace
@@ -448,25 +507,43 @@ struct Builder {
 void build_dag(const Grid& g, const float* xsum, const float* ysum, int mode, const int* tnode, int tn,
                const int64_t* adj_off, const int* tadj, DagNet& out) {
     Builder b(g, xsum, ysum, mode);
-    b.build(tnode, tn, adj_off, tadj, out);
+    DagChunk c;
+    b.build(tnode, tn, adj_off, tadj, c);
+    out.node = c.node;
+    out.child_num = c.child_num;
+    out.depth = c.depth;
+    out.par = c.par;
+    out.slot = c.slot;
+    out.points = c.points;
+    out.par_off.assign(c.node.size() + 1, 0);
+    for (size_t i = 0; i < c.node.size(); i++) out.par_off[i + 1] = out.par_off[i] + c.npar[i];
 }
 
 void build_dags(const Grid& g, const Subnets& s, const int* nets, int64_t n, const float* xsum, const float* ysum,
                 int mode, int threads, DagSet& out) {
-    std::vector<DagNet> per(n);
     if (threads < 1) threads = 1;
-    std::vector<std::thread> pool;
+    const int64_t CH = 512;
+    const int64_t nch = (n + CH - 1) / CH;
+    std::vector<DagChunk> chunks(nch);
+    std::atomic<int64_t> next(0);
     std::vector<std::string> errs(threads);
+    std::vector<std::thread> pool;
     for (int t = 0; t < threads; t++)
         pool.emplace_back([&, t]() {
             try {
-                for (int64_t i = t; i < n; i += threads) {
-                    int net = nets[i];
-                    int64_t b = s.node_off[net];
-                    int tn = (int)(s.node_off[net + 1] - b);
-                    std::vector<int64_t> aoff(tn + 1);
-                    for (int k = 0; k <= tn; k++) aoff[k] = s.adj_off[b + k] - s.adj_off[b];
-                    build_dag(g, xsum, ysum, mode, &s.node[b], tn, aoff.data(), &s.adj[s.adj_off[b]], per[i]);
+                Builder b(g, xsum, ysum, mode);
+                for (;;) {
+                    int64_t c = next.fetch_add(1);
+                    if (c >= nch) break;
+                    DagChunk& ck = chunks[c];
+                    for (auto* v : {&ck.node, &ck.child_num, &ck.depth, &ck.npar}) v->reserve(CH * 64);
+                    for (auto* v : {&ck.par, &ck.slot, &ck.points}) v->reserve(CH * 96);
+                    for (int64_t i = c * CH; i < std::min(n, (c + 1) * CH); i++) {
+                        int net = nets[i];
+                        int64_t b0 = s.node_off[net];
+                        int tn = (int)(s.node_off[net + 1] - b0);
+                        b.build(&s.node[b0], tn, &s.adj_off[b0], &s.adj[s.adj_off[b0]], ck);
+                    }
                 }
             } catch (const std::exception& e) {
                 errs[t] = e.what();
@@ -475,33 +552,57 @@ void build_dags(const Grid& g, const Subnets& s, const int* nets, int64_t n, con
     for (auto& th : pool) th.join();
     for (auto& e : errs)
         if (!e.empty()) throw std::runtime_error(e);
+    // Offsets of every chunk in the final arrays.
+    std::vector<int64_t> cn(nch + 1, 0), ce(nch + 1, 0), cp(nch + 1, 0);
+    for (int64_t c = 0; c < nch; c++) {
+        cn[c + 1] = cn[c] + (int64_t)chunks[c].node.size();
+        ce[c + 1] = ce[c] + (int64_t)chunks[c].par.size();
+        cp[c + 1] = cp[c] + (int64_t)chunks[c].points.size();
+    }
     out = DagSet();
     out.n = n;
     out.node_off.assign(n + 1, 0);
     out.pt_off.assign(n + 1, 0);
-    for (int64_t i = 0; i < n; i++) {
-        out.node_off[i + 1] = out.node_off[i] + (int64_t)per[i].node.size();
-        out.pt_off[i + 1] = out.pt_off[i] + (int64_t)per[i].points.size();
-    }
-    const int64_t N = out.node_off[n];
-    out.node.resize(N);
-    out.child_num.resize(N);
-    out.depth.resize(N);
-    out.par_off.assign(N + 1, 0);
-    out.pt.resize(out.pt_off[n]);
-    for (int64_t i = 0; i < n; i++) {
-        const DagNet& d = per[i];
-        int64_t b = out.node_off[i];
-        for (size_t k = 0; k < d.node.size(); k++) {
-            out.node[b + k] = d.node[k];
-            out.child_num[b + k] = d.child_num[k];
-            out.depth[b + k] = d.depth[k];
-            out.par_off[b + k + 1] = out.par_off[b + k] + (d.par_off[k + 1] - d.par_off[k]);
-        }
-        out.par.insert(out.par.end(), d.par.begin(), d.par.end());
-        out.slot.insert(out.slot.end(), d.slot.begin(), d.slot.end());
-        std::copy(d.points.begin(), d.points.end(), out.pt.begin() + out.pt_off[i]);
-    }
+    out.node.resize(cn[nch]);
+    out.child_num.resize(cn[nch]);
+    out.depth.resize(cn[nch]);
+    out.par_off.resize(cn[nch] + 1);
+    out.par.resize(ce[nch]);
+    out.slot.resize(ce[nch]);
+    out.pt.resize(cp[nch]);
+    next = 0;
+    pool.clear();
+    for (int t = 0; t < threads; t++)
+        pool.emplace_back([&]() {
+            for (;;) {
+                int64_t c = next.fetch_add(1);
+                if (c >= nch) break;
+                const DagChunk& ck = chunks[c];
+                std::copy(ck.node.begin(), ck.node.end(), out.node.begin() + cn[c]);
+                std::copy(ck.child_num.begin(), ck.child_num.end(), out.child_num.begin() + cn[c]);
+                std::copy(ck.depth.begin(), ck.depth.end(), out.depth.begin() + cn[c]);
+                std::copy(ck.par.begin(), ck.par.end(), out.par.begin() + ce[c]);
+                std::copy(ck.slot.begin(), ck.slot.end(), out.slot.begin() + ce[c]);
+                std::copy(ck.points.begin(), ck.points.end(), out.pt.begin() + cp[c]);
+                int64_t e = ce[c];
+                for (size_t k = 0; k < ck.npar.size(); k++) {
+                    out.par_off[cn[c] + k] = e;
+                    e += ck.npar[k];
+                }
+                int64_t nn = cn[c], pp = cp[c];
+                for (size_t k = 0; k < ck.nodes_per_net.size(); k++) {
+                    int64_t i = c * CH + (int64_t)k;
+                    out.node_off[i] = nn;
+                    out.pt_off[i] = pp;
+                    nn += ck.nodes_per_net[k];
+                    pp += ck.points_per_net[k];
+                }
+            }
+        });
+    for (auto& th : pool) th.join();
+    out.par_off[cn[nch]] = ce[nch];
+    out.node_off[n] = cn[nch];
+    out.pt_off[n] = cp[nch];
 }
 
 } // namespace igr

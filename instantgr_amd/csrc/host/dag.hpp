@@ -21,6 +21,12 @@ struct DagSet {
     std::vector<int> node, par, slot, child_num, depth, pt;
 };
 
+// Consecutive nets' DAGs as one builder thread produces them (par local,
+// npar = parent count per node).
+struct DagChunk {
+    std::vector<int> node, child_num, depth, npar, par, slot, points, nodes_per_net, points_per_net;
+};
+
 void build_dag(const Grid& g, const float* xsum, const float* ysum, int mode, const int* tnode, int tn,
                const int64_t* adj_off, const int* tadj, DagNet& out);
 

@@ -49,7 +49,8 @@ class RunStats(C.Structure):
                 ("total_s", C.c_double), ("pattern_batches", C.c_int), ("iterations", C.c_int),
                 ("subnets", C.c_int64), ("ripped", C.c_int64 * MAX_RRR), ("rrr_batches", C.c_int * MAX_RRR),
                 ("rrr_dag_nodes", C.c_int64 * MAX_RRR), ("rrr_view_s", C.c_double * MAX_RRR),
-                ("rrr_dag_build_s", C.c_double * MAX_RRR), ("rrr_route_s", C.c_double * MAX_RRR)]
+                ("rrr_dag_build_s", C.c_double * MAX_RRR), ("rrr_route_s", C.c_double * MAX_RRR),
+                ("rrr_schedule_s", C.c_double * MAX_RRR), ("rrr_prep_s", C.c_double * MAX_RRR)]
 
 
 # Every entry point of include/igr.h with its ctypes signature.
@@ -262,7 +263,8 @@ class Router:
                 "total_s": st.total_s, "pattern_batches": st.pattern_batches, "iterations": it,
                 "subnets": st.subnets, "ripped": list(st.ripped[:it]), "rrr_batches": list(st.rrr_batches[:it]),
                 "rrr_dag_nodes": list(st.rrr_dag_nodes[:it]), "rrr_view_s": list(st.rrr_view_s[:it]),
-                "rrr_dag_build_s": list(st.rrr_dag_build_s[:it]), "rrr_route_s": list(st.rrr_route_s[:it])}
+                "rrr_dag_build_s": list(st.rrr_dag_build_s[:it]), "rrr_route_s": list(st.rrr_route_s[:it]),
+                "rrr_schedule_s": list(st.rrr_schedule_s[:it]), "rrr_prep_s": list(st.rrr_prep_s[:it])}
 
     @property
     def stamp(self):

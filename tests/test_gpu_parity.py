@@ -66,7 +66,7 @@ def test_presum_track_lengths(shape):
         h.update_cost()
         h.compute_presum()
     for a, b in zip(r.costs(), o.costs()):
-        assert np.array_equal(a, b)
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
     r.close()
 
 
@@ -103,17 +103,22 @@ def test_single_batch_bitexact(design_c1):
     r.close()
 
 
-def test_pattern_route_c1_bitexact(design_c1):
+@pytest.mark.parametrize("form", [gpu.FORM_PLAIN, gpu.FORM_BATCHED], ids=["plain", "batched"])
+def test_pattern_route_c1_bitexact(design_c1, form):
     d = design_c1
     r = gpu.Router.from_design(d)
     o = orc.Oracle.from_design(d)
     r.set_nets_from_design(d)
     o.set_nets_from_design(d)
     order = d.array("order")
-    st = r.pattern_route(order)
+    st = r.pattern_route(order, form=form)
     nb = o.pattern_route(order)
     assert st["batches"] == nb
     assert np.array_equal(r.demand().view(np.uint32), o.demand().view(np.uint32))
+    assert np.array_equal(r.timestamp(), o.timestamp())
+    # costs as left by the pass (computed before the last batch)
+    for a, b in zip(r.costs(), o.costs()):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
     goff, gdata = r.routes_flat()
     ooff, odata = o.routes_flat()
     for i in range(r.num_nets):

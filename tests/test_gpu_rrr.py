@@ -43,6 +43,8 @@ def test_rrr_three_passes_c1(design_c1, form):
         assert st["batches"] == nb, it
         assert np.array_equal(r.demand().view(np.uint32), o.demand().view(np.uint32)), it
         assert np.array_equal(r.timestamp(), o.timestamp()), it
+        for a, b in zip(r.costs(), o.costs()):
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), it
         rg = _routes(r)
         ro = [o.route(i) for i in range(o.num_nets)]
         for i in range(len(rg)):
