@@ -1,0 +1,186 @@
+"""Pins of the oracle's DAG rip-up-and-reroute routing (no GPU).
+
+The DAG DP (src/Lshape_route_detour.hpp:123-195, :281-406, :520-694) is
+checked where the mathematics fixes its answer:
+  * a DAG that is a tree with straight edges too short for a layer change
+    reduces to the L-shape DP's model, itself pinned by brute force, so the
+    route it produces costs the L-shape optimum (to the FP32 rounding of the
+    packed keys);
+  * a two-node DAG with a long straight edge is brute-forced over the end
+    layers, the switch layers and the sampled switch points.
+Multi-parent DAGs (detour alternatives) have no closed form: their routes are
+pinned only by the agreement of the HIP kernels with this oracle."""
+import math
+
+import numpy as np
+import pytest
+
+import orc
+
+from test_oracle import _grid
+
+
+def _costs(L, X, Y, DIR, seed, dem):
+    o, cap, w, xlen, ylen = _grid(L, X, Y, DIR, seed=seed)
+    o.set_demand(dem)
+    o.update_cost()
+    o.compute_presum()
+    _, vc, ps = (a.reshape(L, X, Y) for a in o.costs())
+    return o, vc, ps
+
+
+def _route_cost(route, L, X, Y, ps, vc):
+    """Wires by prefix differences, vias by the full cut sum."""
+    XY = X * Y
+    dec = lambda v: (v // XY, v // Y % X, v % Y)
+    cost = 0.0
+    for k in range(1, route[0], 2):
+        a, b = dec(route[k]), dec(route[k + 1])
+        if a[0] == b[0]:
+            cost += ps[a[0], max(a[1], b[1]), max(a[2], b[2])] - ps[a[0], min(a[1], b[1]), min(a[2], b[2])]
+        else:
+            assert a[1:] == b[1:]
+            cost += sum(float(vc[z, a[1], a[2]]) for z in range(min(a[0], b[0]), max(a[0], b[0])))
+    return cost
+
+
+def _tree_dag(nodes, parents, X, Y):
+    n = len(nodes)
+    children = [[] for _ in range(n)]
+    for i in range(1, n):
+        children[parents[i]].append(i)
+    depth = [0] * n
+    for i in range(1, n):
+        depth[i] = depth[parents[i]] + 1
+    par_off = [0]
+    par, slot = [], []
+    for i in range(n):
+        if i > 0:
+            par.append(parents[i])
+            slot.append(children[parents[i]].index(i))
+        par_off.append(len(par))
+    XY = X * Y
+    pts = sorted({v % XY for v in nodes})
+    return {"dag_node_off": np.array([0, n], np.int64), "dag_node": np.array(nodes, np.int32),
+            "dag_par_off": np.array(par_off, np.int64), "dag_par": np.array(par, np.int32),
+            "dag_slot": np.array(slot, np.int32), "dag_child_num": np.array([len(c) for c in children], np.int32),
+            "dag_depth": np.array(depth, np.int32), "dag_pt_off": np.array([0, len(pts)], np.int64),
+            "dag_pt": np.array(pts, np.int32)}
+
+
+def _straight_tree(rng, L, X, Y, n, maxlen):
+    """Random tree whose edges are axis-aligned and at most maxlen long."""
+    XY = X * Y
+    while True:
+        pos = [(int(rng.integers(0, X)), int(rng.integers(0, Y)))]
+        parents = [-1]
+        ok = True
+        while len(pos) < n and ok:
+            p = int(rng.integers(0, len(pos)))
+            px, py = pos[p]
+            for _ in range(50):
+                step = int(rng.integers(1, maxlen + 1)) * (1 if rng.random() < 0.5 else -1)
+                x, y = (px + step, py) if rng.random() < 0.5 else (px, py + step)
+                if 0 <= x < X and 0 <= y < Y and (x, y) not in pos:
+                    pos.append((x, y))
+                    parents.append(p)
+                    break
+            else:
+                ok = False
+        if ok:
+            break
+    layers = [int(rng.choice([0, 1, L])) for _ in range(n)]
+    layers[0] = int(rng.choice([0, 1]))
+    # DFS pre-order
+    order, stack = [], [0]
+    kids = [[] for _ in range(n)]
+    for i in range(1, n):
+        kids[parents[i]].append(i)
+    while stack:
+        v = stack.pop()
+        order.append(v)
+        stack.extend(reversed(kids[v]))
+    ren = {v: i for i, v in enumerate(order)}
+    nodes = [layers[v] * XY + pos[v][0] * Y + pos[v][1] for v in order]
+    par = [-1] + [ren[parents[v]] for v in order[1:]]
+    return nodes, par
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_dag_tree_route_costs_lshape_optimum(seed):
+    rng = np.random.default_rng(500 + seed)
+    L, X, Y, DIR = 4 + seed % 2, 9, 8, seed % 2
+    dem = (rng.integers(0, 10, L * X * Y) * 0.5).astype(np.float32)
+    ref, vc, ps = _costs(L, X, Y, DIR, seed, dem)
+    nodes, parents = _straight_tree(rng, L, X, Y, 2 + seed % 4, 4)
+    ref.set_nets(np.array([0, len(nodes)], np.int64), np.array(nodes, np.int32), np.array(parents, np.int32))
+    best, lroute = ref.lshape_probe(0)
+    o, _, _ = _costs(L, X, Y, DIR, seed, dem)
+    o.set_nets(np.array([0, len(nodes)], np.int64), np.array(nodes, np.int32), np.array(parents, np.int32))
+    nb = orc.dag_pass(o, np.array([0], np.int32), _tree_dag(nodes, parents, X, Y))
+    assert nb == 1
+    route = o.route(0)
+    got = _route_cost(route, L, X, Y, ps, vc)
+    assert math.isclose(got, best, rel_tol=1e-6), (got, best)
+    assert math.isclose(_route_cost(lroute, L, X, Y, ps, vc), best, rel_tol=1e-12)
+
+
+def _two_node_brute(L, X, Y, DIR, ps, vc, a, b):
+    """Minimum cost of connecting node b (child) to node a (root) along one
+    straight edge: same-layer wire, or (for an edge longer than 4) a layer
+    change between layers of the same parity at a sampled switch point,
+    plus the via stacks at both ends over the layers meeting there."""
+    XY = X * Y
+    (la, ax, ay), (lb, bx, by) = [(v // XY, v // Y % X, v % Y) for v in (a, b)]
+    dirl = lambda l: (l & 1) ^ DIR
+    vert = ax == bx
+    dist = abs(ax - bx) + abs(ay - by)
+    wseg = lambda l, p, q: ps[l, max(p[0], q[0]), max(p[1], q[1])] - ps[l, min(p[0], q[0]), min(p[1], q[1])]
+    stack = lambda ls, x, y: sum(float(vc[z, x, y]) for z in range(min(ls), max(ls)))
+    best = math.inf
+    for fl in range(L):           # layer at the child end
+        if dirl(fl) != (1 if vert else 0):
+            continue
+        for tl in range(L):       # layer at the parent end
+            if (tl & 1) != (fl & 1):
+                continue
+            if fl == tl:
+                edge = wseg(fl, (bx, by), (ax, ay))
+            elif dist > 4:
+                dd = dist // min(dist, 20)
+                s0, s1 = (min(ay, by), max(ay, by)) if vert else (min(ax, bx), max(ax, bx))
+                edge = math.inf
+                for sp in range(s0 + dd, s1, dd):
+                    sw = (bx, sp) if vert else (sp, by)
+                    c = stack([fl, tl], *sw) + wseg(fl, (bx, by), sw) + wseg(tl, sw, (ax, ay))
+                    edge = min(edge, c)
+            else:
+                continue
+            lsb = [fl] + ([lb] if lb < L else [])
+            lsa = [tl] + ([la] if la < L else [])
+            best = min(best, edge + stack(lsb, bx, by) + stack(lsa, ax, ay))
+    return best
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_dag_two_node_layer_change_brute_force(seed):
+    rng = np.random.default_rng(700 + seed)
+    L, X, Y, DIR = 5, 30, 26, seed % 2
+    # congest the lower layers so that switching layers pays off
+    dem = (rng.integers(0, 6, (L, X, Y)) * 0.5).astype(np.float32)
+    dem[:2] += rng.integers(3, 9, (2, X, Y)).astype(np.float32)
+    ref, vc, ps = _costs(L, X, Y, DIR, seed, dem.ravel())
+    XY = X * Y
+    ax, ay = int(rng.integers(0, X)), int(rng.integers(0, Y))
+    if seed % 4 < 2:
+        bx, by = ax, int((ay + rng.integers(5, Y - 1)) % Y)
+    else:
+        bx, by = int((ax + rng.integers(5, X - 1)) % X), ay
+    la, lb = int(rng.choice([0, 1, L])), int(rng.choice([0, 1, L]))
+    nodes, parents = [la * XY + ax * Y + ay, lb * XY + bx * Y + by], [-1, 0]
+    brute = _two_node_brute(L, X, Y, DIR, ps, vc, nodes[0], nodes[1])
+    o, _, _ = _costs(L, X, Y, DIR, seed, dem.ravel())
+    o.set_nets(np.array([0, 2], np.int64), np.array(nodes, np.int32), np.array(parents, np.int32))
+    orc.dag_pass(o, np.array([0], np.int32), _tree_dag(nodes, parents, X, Y))
+    got = _route_cost(o.route(0), L, X, Y, ps, vc)
+    assert math.isclose(got, brute, rel_tol=1e-6), (got, brute)
