@@ -7,7 +7,12 @@ on a fresh device context.  Both execution forms run in the same process:
 per-depth DAG launches, host synchronisation per step) and 'batched'; their
 routes must be identical and their evaluator scores equal.
 
-Prints one JSON line (rank 0).  Only one GPU is supported.
+Prints one JSON line.  Only one GPU is supported.
+
+Helpers for the committed profile summaries (no GPU needed):
+  --summarize-stats DIR             kernel table from rocprofv3 --kernel-trace --stats
+  --summarize-pmc FETCH_DIR WRITE_DIR  calibrated HBM bytes per dispatch from two
+                                    rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE)
 """
 import argparse
 import json
@@ -19,14 +24,12 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Algorithmic HBM bytes per grid cell of one launch of each streaming kernel.
-BYTES_PER_CELL = {
-    "cost_update": 16,   # read capacity + demand, write wcost + vcost
-    "presum": 12,        # read wcost (f32), write presum (f64)
-    "wire_commit": 16,   # read difference array, read/write demand, clear
-    "track_fused": 36,   # commit + cost + prefix of a track in one pass
-}
 HBM_PEAK_GBS = 8000.0
+# Kernel class -> kernel symbol (the name rocprofv3 reports).
+KERNEL_SYMBOL = {"dag_dp": "k_dag_net", "dag_edges": "k_dag_edges_pairs", "track_fused": "k_track_fused",
+                 "lshape_dp": "k_lshape_plain", "cost_update": "k_update_wcost", "presum": "k_presum",
+                 "wire_commit": "k_commit_edges"}
+PROFILES = os.path.join(ROOT, "profiles")
 
 
 def parse():
@@ -40,6 +43,12 @@ def parse():
     ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--forms", default="plain,batched")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a 1 GiB device copy first (bandwidth reference, counter calibration kernel)")
+    ap.add_argument("--summarize-stats", metavar="DIR",
+                    help="print a summary of the rocprofv3 kernel statistics found under DIR and exit")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="print the calibrated HBM traffic per dispatch from two rocprofv3 --pmc runs and exit")
     return ap.parse_args()
 
 
@@ -59,42 +68,56 @@ def one_step(design, form, rrr, threads, profile=False):
     return r, st
 
 
-def roofline(kt, cells):
-    """Dominant kernel class and its achieved bandwidth on algorithmic bytes."""
-    dom = max(kt.items(), key=lambda kv: kv[1][0])
-    name, (ms, launches) = dom
-    out = {"kernel": name, "total_ms": ms, "launches": launches}
-    if name in BYTES_PER_CELL and launches:
-        per_launch = BYTES_PER_CELL[name] * cells
-        achieved = per_launch * launches / (ms * 1e-3) / 1e9
-        out.update({"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                    "bytes_per_launch": per_launch, "avg_launch_us": ms * 1e3 / launches})
+def _traffic(kernel):
+    """HBM bytes per dispatch of 'kernel' from the committed counter summary
+    (lines 'traffic <kernel> <bytes>'), or None."""
+    path = os.path.join(PROFILES, "traffic_config1_batched.txt")
+    if not kernel or not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic" and f[1] == kernel:
+            return float(f[2])
+    return None
+
+
+def roofline(kt, kb):
+    """Dominant kernel class (by device time in the timed step) and its
+    achieved bandwidth on algorithmic bytes; every class with a byte model is
+    listed beside it."""
+    name, (ms, launches) = max(kt.items(), key=lambda kv: kv[1][0])
+    out = {"kernel": name, "symbol": KERNEL_SYMBOL.get(name), "total_ms": ms, "launches": launches,
+           "bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    b = kb.get(name, 0.0)
+    if b > 0 and ms > 0:
+        achieved = b / (ms * 1e-3) / 1e9
+        out.update({"achieved": achieved, "frac": achieved / HBM_PEAK_GBS, "bytes_per_launch": b / launches,
+                    "avg_launch_us": ms * 1e3 / launches})
     else:
-        out.update({"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
-                    "traffic": None})
-    # Streaming grid kernels, reported beside the dominant one.
-    grid = {}
-    for k, per in BYTES_PER_CELL.items():
-        ms_k, n_k = kt.get(k, (0.0, 0))
+        out.update({"achieved": None, "frac": None})
+    out["traffic"] = _traffic(KERNEL_SYMBOL.get(name))
+    classes = {}
+    for k, (ms_k, n_k) in kt.items():
         if n_k and ms_k > 0:
-            grid[k] = {"GB/s": per * cells * n_k / (ms_k * 1e-3) / 1e9, "avg_launch_us": ms_k * 1e3 / n_k,
-                       "launches": n_k}
-    out["grid_kernels"] = grid
+            e = {"ms": round(ms_k, 3), "launches": n_k, "avg_launch_us": round(ms_k * 1e3 / n_k, 2)}
+            if kb.get(k, 0) > 0:
+                e["GB/s"] = round(kb[k] / (ms_k * 1e-3) / 1e9, 1)
+            classes[k] = e
+    out["classes"] = classes
     return out
 
 
 def cpu_baseline():
     """The CPU oracle (sequential restatement of the reference logic, one
     thread) routing a bounded sample of the same workload: the generator's
-    configuration-1 distributions at the same net density on a 128x128 grid
-    (12,800 nets), pattern pass plus 3 rip-up-and-reroute passes."""
+    configuration-1 distributions at the same net density on a 176x176 grid
+    (24,200 nets), tree build + pattern pass + 3 rip-up-and-reroute passes."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
     from instantgr_amd.host import Design
-    nets = 12800
-    d = Design.custom(layers=10, xsize=128, ysize=128, nets=nets, pin_cap=64, seed=2)
+    nets = 24200
+    d = Design.custom(layers=10, xsize=176, ysize=176, nets=nets, pin_cap=64, seed=2)
     t0 = time.perf_counter()
     d.build(1)
     o = orc.Oracle.from_design(d)
@@ -111,19 +134,103 @@ def cpu_baseline():
         dg = d.build_dags(sel, xs, ys, 0, 1)
         orc.dag_pass(o, sel, dg, 0)
     dt = time.perf_counter() - t0
+    cpu = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": nets / dt, "unit": "nets/s", "cores": 1, "kind": "port",
-            "sample": "10 layers, 128x128 gcells, 12800 nets (configuration-1 generator and density), "
-                      "tree build + pattern pass + 3 RRR passes; %.1f s" % dt}
+            "sample": "10 layers, 176x176 gcells, 24200 nets (configuration-1 generator and net density), "
+                      "tree build + pattern pass + 3 RRR passes, one thread; %.1f s on %s (%d host cores)"
+                      % (dt, cpu or "unknown CPU", os.cpu_count() or 0)}
+
+
+def _find(root, suffix):
+    hits = []
+    for dp, _, files in os.walk(root):
+        hits += [os.path.join(dp, f) for f in files if f.endswith(suffix)]
+    return sorted(hits)
+
+
+def _short(name, with_template=False):
+    """Kernel symbol out of a demangled name (namespaces and parameters
+    dropped; template arguments kept on request)."""
+    import re
+    m = re.search(r"\b(k_[A-Za-z0-9_]+|__amd_[A-Za-z0-9_]+)(<[^()]*?>)?\(", name)
+    if not m:
+        return name[:40]
+    return m.group(1) + ((m.group(2) or "") if with_template else "")
+
+
+def summarize_stats(root):
+    import csv
+    rows = []
+    for path in _find(root, "kernel_stats.csv"):
+        with open(path) as f:
+            rows += list(csv.DictReader(f))
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    print("%-44s %8s %12s %10s %7s" % ("kernel", "calls", "total_ms", "avg_us", "pct"))
+    for r in rows[:30]:
+        print("%-44s %8s %12.3f %10.2f %7.2f" % (_short(r["Name"], True)[:44], r["Calls"],
+                                                  float(r["TotalDurationNs"]) / 1e6,
+                                                  float(r["AverageNs"]) / 1e3,
+                                                  100 * float(r["TotalDurationNs"]) / tot))
+    print("%-44s %8s %12.3f" % ("all kernels", "", tot / 1e6))
+
+
+def summarize_pmc(fetch_root, write_root, copy_bytes=1 << 30):
+    """Average FETCH_SIZE / WRITE_SIZE (KiB) per dispatch of each kernel, each
+    scaled by what the same counter reads for k_copy_probe, which reads and
+    writes exactly copy_bytes per dispatch."""
+    import csv
+    from collections import defaultdict
+
+    def load(root, counter):
+        acc = defaultdict(list)
+        for path in _find(root, "counter_collection.csv"):
+            with open(path) as f:
+                for r in csv.DictReader(f):
+                    if r.get("Counter_Name") == counter:
+                        acc[_short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+        return {k: sum(v) / len(v) for k, v in acc.items()}, {k: len(v) for k, v in acc.items()}
+    fetch, nf = load(fetch_root, "FETCH_SIZE")
+    write, _ = load(write_root, "WRITE_SIZE")
+    if "k_copy_probe" not in fetch or "k_copy_probe" not in write:
+        raise SystemExit("k_copy_probe missing from the counter runs (run bench.py with --copy-probe)")
+    ff = copy_bytes / (fetch["k_copy_probe"] * 1024.0)
+    fw = copy_bytes / (write["k_copy_probe"] * 1024.0)
+    print("# calibration: k_copy_probe reads and writes %d B per dispatch; FETCH_SIZE %.1f KiB, WRITE_SIZE %.1f KiB"
+          " -> factors %.4f (fetch), %.4f (write)" % (copy_bytes, fetch["k_copy_probe"], write["k_copy_probe"],
+                                                      ff, fw))
+    print("%-24s %10s %16s %16s %18s" % ("kernel", "dispatches", "FETCH_KiB_avg", "WRITE_KiB_avg",
+                                         "traffic_B_per_disp"))
+    lines = []
+    for k in sorted(fetch, key=lambda k: -fetch[k] * nf[k]):
+        t = fetch[k] * 1024.0 * ff + write.get(k, 0.0) * 1024.0 * fw
+        print("%-24s %10d %16.1f %16.1f %18.0f" % (k, nf[k], fetch[k], write.get(k, 0.0), t))
+        lines.append("traffic %s %.0f" % (k, t))
+    print("\n".join(lines))
 
 
 def main():
     a = parse()
+    if a.summarize_stats:
+        summarize_stats(a.summarize_stats)
+        return
+    if a.summarize_pmc:
+        summarize_pmc(*a.summarize_pmc)
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU only: the routing path runs on a single MI355X "
               "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
         sys.exit(2)
     from instantgr_amd import gpu
     from instantgr_amd.host import Design
+    copy_gbs = gpu.copy_probe(1 << 30, 20) if a.copy_probe else None
     design = Design.preset(a.config, nets=a.nets)
     forms = [f.strip() for f in a.forms.split(",") if f.strip()]
     form_id = {"plain": gpu.FORM_PLAIN, "batched": gpu.FORM_BATCHED}
@@ -139,12 +246,13 @@ def main():
             r, st = one_step(design, form_id[f], a.rrr, a.threads, profile=(k == a.steps - 1))
             times.append(st["step_s"])
             if last is not None:
-                last.close()
+                last[0].close()
             last = (r, st)
         r, st = last
         kt = r.kernel_times()
+        kb = r.kernel_bytes()
         routes[f] = r.routes_flat()
-        results[f] = {"step_s": times, "stats": st, "kernels": kt}
+        results[f] = {"step_s": times, "stats": st, "kernels": kt, "bytes": kb}
         print("[bench] %s: %s" % (f, json.dumps({"step_s": times, "pattern_batches": st["pattern_batches"],
                                                    "ripped": st["ripped"], "tree_s": st["tree_s"],
                                                    "pattern_s": st["pattern_s"],
@@ -159,7 +267,6 @@ def main():
 
     info = design.info
     nets_in = info.nets
-    cells = info.L * info.X * info.Y
     # Identical routes and score across forms.
     import numpy as np
     same = None
@@ -182,16 +289,18 @@ def main():
         "value": value, "unit": "nets/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": step_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "configuration %d: %d layers, %dx%d gcells, %d nets; tree build + pattern pass + "
-                               "%d rip-up-and-reroute passes" % (a.config, info.metal_layers, info.X, info.Y,
-                                                                 nets_in, a.rrr),
+        "config": {"workload": "configuration %d: %d metal layers, %dx%d gcells, %d nets; tree build + pattern "
+                               "pass + %d rip-up-and-reroute passes" % (a.config, info.metal_layers, info.X, info.Y,
+                                                                        nets_in, a.rrr),
                    "subnets": int(info.subnets), "form": head},
         "forms": {f: {"nets_per_s": nets_in / (sum(v["step_s"]) / len(v["step_s"])),
                       "ms_per_step": 1e3 * sum(v["step_s"]) / len(v["step_s"]),
                       "score": scores[f]} for f, v in results.items()},
         "routes_identical": same,
-        "roofline": roofline(results[head]["kernels"], cells),
+        "roofline": roofline(results[head]["kernels"], results[head]["bytes"]),
     }
+    if copy_gbs is not None:
+        out["roofline"]["copy_probe_GBs"] = copy_gbs
     if "plain" in results and "batched" in results:
         out["speedup_batched_vs_plain"] = out["forms"]["plain"]["ms_per_step"] / out["forms"]["batched"]["ms_per_step"]
     out["cpu_baseline"] = None if a.no_cpu else cpu_baseline()

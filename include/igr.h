@@ -166,9 +166,18 @@ int igr_congestion_view(igr_ctx* ctx, float* xsum, float* ysum);
 #define IGR_KCLASSES 11
 int igr_set_profiling(igr_ctx* ctx, int on);
 int igr_kernel_times(igr_ctx* ctx, double* ms, int64_t* launches);
+/* Algorithmic HBM bytes moved by the launches recorded since profiling was
+ * enabled, per kernel class (IGR_KCLASSES entries; 0 where no byte model is
+ * defined).  The per-unit figures are stated beside each launcher. */
+int igr_kernel_bytes(igr_ctx* ctx, double* bytes);
 
 /* Current commit timestamp counter. */
 int igr_stamp(igr_ctx* ctx);
+
+/* Bandwidth reference: a plain 16-byte-vector device copy of 'bytes',
+ * repeated 'reps' times; *gbs = (bytes read + written) / time.  Also the
+ * calibration kernel for the memory traffic counters. */
+int igr_copy_probe(int64_t bytes, int reps, double* gbs);
 
 /* The routing exp evaluated on the device (diagnostic). */
 int igr_expf(const float* in, float* out, int64_t n);

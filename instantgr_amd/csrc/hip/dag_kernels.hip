@@ -623,12 +623,12 @@ void launch_dag_batch_plain(igr_ctx* c, const DagBatch& b, int stamp, const std:
     HIP_OK(hipGetLastError());
     DagArgs a = make_args(c, b, stamp);
     if (ne > 0) {
-        KScope ks(c, K_DAG_EDGES);
+        KScope ks(c, K_DAG_EDGES, b.bytes_edges);
         k_dag_edges<<<blocks(ne * c->g.L), TPB, 0, s>>>(a, ne);
         HIP_OK(hipGetLastError());
     }
     {
-        KScope ks(c, K_DAG_DP);
+        KScope ks(c, K_DAG_DP, b.bytes_dp);
         for (int d = b.levels - 1; d >= 0; d--) {
             int64_t lo = h_lvl_off[b.lvl0 + d], hi = h_lvl_off[b.lvl0 + d + 1];
             if (hi > lo) k_dag_dp<<<blocks(hi - lo), TPB, 0, s>>>(a, D.lvl_nodes.p, lo, hi);
@@ -651,13 +651,13 @@ void launch_dag_batch_wave(igr_ctx* c, const DagBatch& b, int stamp) {
     DagArgs a = make_args(c, b, stamp);
     const int64_t ne = b.edge1 - b.edge0;
     if (ne > 0) {
-        KScope ke(c, K_DAG_EDGES);
+        KScope ke(c, K_DAG_EDGES, b.bytes_edges);
         const int64_t items = ne * c->g.L * c->g.L;
         k_dag_edges_pairs<<<blocks(items), TPB, 0, c->stream>>>(a, ne);
         HIP_OK(hipGetLastError());
     }
     NetLevels lv{D.nl_off.p, D.nlevels.p, D.lvl_node_off.p, D.lvl_edge_off.p, D.dnode.p, D.dedge.p};
-    KScope ks(c, K_DAG_DP);
+    KScope ks(c, K_DAG_DP, b.bytes_dp);
     k_dag_net<<<(unsigned)n, WTPB, 0, c->stream>>>(a, lv, D.nets.p, b.net0, b.net1, D.node_off.p, D.route_slot.p,
                                                     c->nets.dag_routes.back()->p, c->nets.rptr.p);
     HIP_OK(hipGetLastError());

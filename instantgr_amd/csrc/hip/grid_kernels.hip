@@ -269,8 +269,8 @@ int tracks(const GridDev& g) {
     } while (0)
 
 void launch_update_cost(igr_ctx* c) {
-    KScope ks(c, K_COST);
     int64_t n = c->cells();
+    KScope ks(c, K_COST, 16.0 * n);   // read cap, demand; write wcost, vcost
     k_update_wcost<<<blocks(n), TPB, 0, c->stream>>>(c->g);
     HIP_OK(hipGetLastError());
     k_update_vcost<<<blocks(n), TPB, 0, c->stream>>>(c->g);
@@ -278,12 +278,14 @@ void launch_update_cost(igr_ctx* c) {
 }
 
 void launch_presum(igr_ctx* c) {
-    KScope ks(c, K_PRESUM);
+    KScope ks(c, K_PRESUM, 12.0 * c->cells());   // read wcost f32, write presum f64
     DISPATCH_K(std::max(c->g.X, c->g.Y), k_presum, tracks(c->g), c->stream, c->g);
 }
 
 void launch_wire_commit(igr_ctx* c, int stamp) {
-    KScope ks(c, K_WIRE_COMMIT);
+    // scan: read/write the difference array; commit: read it, read/write
+    // demand, write timestamps where covered (not counted); clear it
+    KScope ks(c, K_WIRE_COMMIT, 16.0 * c->cells());
     DISPATCH_K(std::max(c->g.X, c->g.Y), k_scan_int, tracks(c->g), c->stream, c->g, c->g.pre);
     int64_t n = c->cells();
     k_commit_edges<<<blocks(n), TPB, 0, c->stream>>>(c->g, stamp);
@@ -367,6 +369,38 @@ void launch_congestion(igr_ctx* c, float* d_xsum, float* d_ysum) {
 void launch_expf(const float* in, float* out, int64_t n, hipStream_t s) {
     k_expf<<<blocks(n), TPB, 0, s>>>(in, out, n);
     HIP_OK(hipGetLastError());
+}
+
+namespace {
+// Plain 16-byte-vector copy: the bandwidth reference and the calibration
+// kernel for the memory counters (bytes read = bytes written = n * 16).
+__global__ void __launch_bounds__(TPB) k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                    int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) dst[i] = src[i];
+}
+} // namespace
+
+double copy_probe(int64_t bytes, int reps, hipStream_t s) {
+    const int64_t n = bytes / 16;
+    if (n <= 0 || reps <= 0) throw std::invalid_argument("copy probe needs bytes >= 16 and reps >= 1");
+    DevBuf<float4> a, b;
+    a.zero(n, s);
+    b.alloc(n);
+    const unsigned grid = (unsigned)std::min<int64_t>((n + TPB - 1) / TPB, 256 * 64);
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    k_copy_probe<<<grid, TPB, 0, s>>>(a.p, b.p, n);   // warm-up
+    HIP_OK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; r++) k_copy_probe<<<grid, TPB, 0, s>>>(a.p, b.p, n);
+    HIP_OK(hipEventRecord(e1, s));
+    HIP_OK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIP_OK(hipGetLastError());
+    return 2.0 * 16.0 * (double)n * reps / (ms * 1e-3) / 1e9;
 }
 
 } // namespace igr_dev

@@ -165,6 +165,22 @@ int igr_set_nets(igr_ctx* c, const igr_net_desc* d) {
             N.max_nodes = std::max<int>(N.max_nodes, (int)k);
             N.h_route_off[i + 1] = N.h_route_off[i] + std::max<int64_t>(3, 8 * k - 5);
         }
+        // Algorithmic bytes of each net's L-shape DP: per node its position,
+        // parent, via-stack costs (4 B per layer) and route output; per
+        // straight edge two prefix sums per layer; per L-shaped edge four
+        // prefix sums per layer plus the corner via costs.
+        N.h_lbytes.assign(n, 0.0);
+        const int Lr = c->g.L;
+        for (int64_t i = 0; i < n; i++) {
+            const int64_t b = N.h_node_off[i], k = N.h_node_off[i + 1] - b;
+            int64_t straight = 0;
+            for (int64_t j = 1; j < k; j++) {
+                const int v = d->nodes[b + j], u = d->nodes[b + d->parents[b + j]];
+                if ((v % XY) / c->g.Y == (u % XY) / c->g.Y || v % c->g.Y == u % c->g.Y) straight++;
+            }
+            N.h_lbytes[i] = (double)k * (8 + 4 * Lr + 32) + (double)straight * 8 * Lr +
+                            (double)(k - 1 - straight) * 40 * Lr;
+        }
         hipStream_t s = c->stream;
         N.node_off.upload(d->node_off, n + 1, s);
         N.node.upload(d->nodes, N.h_node_off[n], s);
@@ -230,11 +246,11 @@ void ensure_scratch(igr_ctx* c, int64_t nodes) {
 }
 
 // Device work of one plain batch: the reference sequence of launches.
-void plain_batch(igr_ctx* c, const int* d_nets, const int64_t* d_off, int64_t n) {
+void plain_batch(igr_ctx* c, const int* d_nets, const int64_t* d_off, int64_t n, double lbytes = 0) {
     int stamp = ++c->stamp;
     launch_update_cost(c);
     launch_presum(c);
-    launch_lshape_plain(c, d_nets, d_off, n, stamp);
+    launch_lshape_plain(c, d_nets, d_off, n, stamp, lbytes);
     launch_wire_commit(c, stamp);
     launch_via_commit(c, d_nets, n, stamp, 1);
 }
@@ -311,9 +327,13 @@ int igr_pattern_route(igr_ctx* c, const int* order, int64_t n, int form, igr_pas
         ensure_scratch(c, max_nodes);
         c->scratch.net_list.upload(list.data(), n, c->stream);
         c->scratch.scr_off.upload(off.data(), n, c->stream);
+        std::vector<double> lbytes(nb, 0.0);
+        for (int b = 0; b < nb; b++)
+            for (int64_t i = cnt[b]; i < cnt[b + 1]; i++) lbytes[b] += c->nets.h_lbytes[list[i]];
         if (form == IGR_FORM_PLAIN) {
             for (int b = 0; b < nb; b++)
-                plain_batch(c, c->scratch.net_list.p + cnt[b], c->scratch.scr_off.p + cnt[b], cnt[b + 1] - cnt[b]);
+                plain_batch(c, c->scratch.net_list.p + cnt[b], c->scratch.scr_off.p + cnt[b], cnt[b + 1] - cnt[b],
+                            lbytes[b]);
         } else {
             // Costs of every track once, then per batch: DP (scattering wires,
             // vias and dirty flags), then the fused commit + cost pass over the
@@ -324,7 +344,7 @@ int igr_pattern_route(igr_ctx* c, const int* order, int64_t n, int form, igr_pas
             for (int b = 0; b < nb; b++) {
                 const int stamp = ++c->stamp;
                 launch_lshape_plain(c, c->scratch.net_list.p + cnt[b], c->scratch.scr_off.p + cnt[b],
-                                    cnt[b + 1] - cnt[b], stamp);
+                                    cnt[b + 1] - cnt[b], stamp, lbytes[b]);
                 launch_track_fused(c, stamp, 0, b + 1 < nb);
             }
             HIP_OK(hipStreamSynchronize(c->stream));
@@ -437,12 +457,47 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
             }
             for (auto& x : th) x.join();
         };
+        // Algorithmic bytes (HBM) of the DAG kernels per routed net.
+        //  edge layer changes, per edge longer than 4 and per (from, to) layer
+        //  pair of its direction: at each sampled switch point two prefix
+        //  sums (16 B) and the via costs between the layers (4 B each), plus
+        //  the result (12 B);
+        //  DP and extraction: per node position, via-stack costs, layer costs
+        //  written and read back, route output (32 B); per edge parent and
+        //  slot, same-layer prefix pair per layer of its direction, the packed
+        //  key atomics per layer; per long edge the layer-change costs read;
+        //  per child slot the keys read and the back-pointers written.
+        double pairs[2] = {0, 0}, pvia[2] = {0, 0};
+        for (int fl = 0; fl < L; fl++)
+            for (int tl = fl & 1; tl < L; tl += 2) {
+                pairs[layer_dir(fl, c->g.DIR)] += 1;
+                pvia[layer_dir(fl, c->g.DIR)] += std::abs(fl - tl);
+            }
+        std::vector<double> ebytes(n, 0.0), dbytes(n, 0.0);
         pfor(n, [&](int64_t a0, int64_t a1) {
             for (int64_t r = a0; r < a1; r++) {
                 const int64_t i = rorder[r], b0 = dg->node_off[i], b1 = dg->node_off[i + 1];
                 int64_t slots = 0;
                 int maxd = 0;
                 for (int64_t k = b0; k < b1; k++) { slots += dg->child_num[k]; maxd = std::max(maxd, dg->depth[k]); }
+                double eb = 0, longp = 0;
+                for (int64_t k = b0; k < b1; k++) {
+                    const int v = dg->nodes[k];
+                    for (int64_t q = dg->par_off[k]; q < dg->par_off[k + 1]; q++) {
+                        const int u = dg->nodes[b0 + dg->par[q]];
+                        const int vx = v % XY / c->g.Y, vy = v % c->g.Y, ux = u % XY / c->g.Y, uy = u % c->g.Y;
+                        const int dist = std::abs(vx - ux) + std::abs(vy - uy);
+                        if (dist <= 4 || (vx != ux && vy != uy)) continue;
+                        const int dir = vx == ux ? 1 : 0;
+                        const int dd = dist / std::min(dist, 20);
+                        const int ns = (dist - 1) / dd;
+                        eb += ns * (16 * pairs[dir] + 4 * pvia[dir]) + 12 * pairs[dir];
+                        longp += pairs[dir];
+                    }
+                }
+                const int64_t nn = b1 - b0, ne = dg->par_off[b1] - dg->par_off[b0];
+                ebytes[r] = eb;
+                dbytes[r] = nn * (4.0 + 4 * L + 16 * ML + 32) + ne * (8.0 + 8 * L + 8 * ML) + longp * 8 + slots * 16.0 * ML;
                 nbase[r + 1] = b1 - b0;
                 ebase[r + 1] = dg->par_off[b1] - dg->par_off[b0];
                 sbase[r + 1] = slots;
@@ -524,7 +579,11 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
             B.slot0 = sbase[B.net0];
             B.slot1 = sbase[B.net1];
             int maxd = 0;
-            for (int64_t r = B.net0; r < B.net1; r++) maxd = std::max<int>(maxd, (int)(lbase[r + 1] - lbase[r]) - 1);
+            for (int64_t r = B.net0; r < B.net1; r++) {
+                maxd = std::max<int>(maxd, (int)(lbase[r + 1] - lbase[r]) - 1);
+                B.bytes_edges += ebytes[r];
+                B.bytes_dp += dbytes[r];
+            }
             B.levels = maxd + 1;
             B.lvl0 = (int64_t)lvl_off.size() - 1;
             if (!batched) {
@@ -663,6 +722,20 @@ int igr_set_profiling(igr_ctx* c, int on) {
         HIP_OK(hipStreamSynchronize(c->stream));
         c->prof.reset();
         c->prof.on = on != 0;
+        HIP_OK(hipMemset(c->track_stats.p, 0, 2 * sizeof(unsigned long long)));
+    });
+}
+
+int igr_kernel_bytes(igr_ctx* c, double* bytes) {
+    return guard([&]() {
+        HIP_OK(hipStreamSynchronize(c->stream));
+        for (int k = 0; k < K_NCLASS; k++) bytes[k] = c->prof.bytes[k];
+        // fused track pass: 36 B per processed cell (read demand, capacity,
+        // timestamp, wire differences, via cuts; write wcost, vcost, presum),
+        // 12 B more where a rip-up is applied (its three difference arrays)
+        unsigned long long st[2] = {0, 0};
+        HIP_OK(hipMemcpy(st, c->track_stats.p, sizeof(st), hipMemcpyDeviceToHost));
+        bytes[K_TRACK_FUSED] = 36.0 * (double)st[0] + 12.0 * (double)st[1];
     });
 }
 
@@ -677,6 +750,20 @@ int igr_kernel_times(igr_ctx* c, double* ms, int64_t* launches) {
             ms[c->prof.cls[i]] += t;
             launches[c->prof.cls[i]]++;
         }
+    });
+}
+
+int igr_copy_probe(int64_t bytes, int reps, double* gbs) {
+    return guard([&]() {
+        hipStream_t s;
+        HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        try {
+            *gbs = copy_probe(bytes, reps, s);
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        HIP_OK(hipStreamDestroy(s));
     });
 }
 

@@ -62,6 +62,7 @@ struct NetsDev {
     std::vector<int> h_node;          // host copies used by the host-side batch logic
     std::vector<int*> h_rptr;
     std::vector<int64_t> h_rcap;      // capacity of each net's current route slot
+    std::vector<double> h_lbytes;     // algorithmic bytes of each net's L-shape DP
     std::vector<DevBuf<int>*> dag_routes;
     int max_nodes = 0;
     ~NetsDev() { clear_dag_routes(); }
@@ -114,6 +115,7 @@ struct KProf {
     size_t used = 0;
     std::vector<int> cls;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> rec;
+    double bytes[16] = {};         // algorithmic HBM bytes per kernel class
     hipEvent_t get() {
         if (used == pool.size()) {
             hipEvent_t e;
@@ -126,6 +128,7 @@ struct KProf {
         used = 0;
         cls.clear();
         rec.clear();
+        for (double& b : bytes) b = 0;
     }
     ~KProf() {
         for (auto e : pool) (void)hipEventDestroy(e);
@@ -158,15 +161,17 @@ struct igr_ctx {
 };
 
 namespace igr_dev {
-// Records the device time of the launches made during its lifetime.
+// Records the device time of the launches made during its lifetime and adds
+// their algorithmic HBM bytes to the class total.
 struct KScope {
     igr_ctx* c;
     int k;
     hipEvent_t a = nullptr;
-    KScope(igr_ctx* c_, int k_) : c(c_), k(k_) {
+    KScope(igr_ctx* c_, int k_, double bytes = 0) : c(c_), k(k_) {
         if (c->prof.on) {
             a = c->prof.get();
             HIP_OK(hipEventRecord(a, c->stream));
+            c->prof.bytes[k] += bytes;
         }
     }
     ~KScope() {
@@ -185,7 +190,8 @@ void launch_update_cost(igr_ctx* c);
 void launch_presum(igr_ctx* c);
 void launch_wire_commit(igr_ctx* c, int stamp);
 void launch_via_commit(igr_ctx* c, const int* d_nets, int64_t n, int stamp, int K);
-void launch_lshape_plain(igr_ctx* c, const int* d_nets, const int64_t* d_scr_off, int64_t n, int stamp);
+void launch_lshape_plain(igr_ctx* c, const int* d_nets, const int64_t* d_scr_off, int64_t n, int stamp,
+                         double bytes = 0);
 void launch_to_ref(igr_ctx* c, const void* src, void* dst, int elem_bytes);
 void launch_from_ref(igr_ctx* c, const void* src, void* dst, int elem_bytes);
 void launch_ripup_mark(igr_ctx* c, int th, uint8_t* d_is_of, int* d_of);
@@ -201,12 +207,16 @@ struct DagBatch {
     int64_t net0, net1, node0, node1, edge0, edge1, slot0, slot1;
     int levels;
     int64_t lvl0;               // first entry of this batch in lvl_off
+    double bytes_edges = 0, bytes_dp = 0;   // algorithmic HBM bytes
 };
 void launch_dag_batch_plain(igr_ctx* c, const DagBatch& b, int stamp, const std::vector<int64_t>& h_lvl_off);
 void launch_dag_compact(igr_ctx* c, const DagBatch& b);
 // Batched form: one workgroup per net, all depth levels in one launch.
 void launch_dag_batch_wave(igr_ctx* c, const DagBatch& b, int stamp);
 void launch_expf(const float* in, float* out, int64_t n, hipStream_t s);
+// Device-to-device copy of 'bytes' repeated 'reps' times; returns GB/s
+// (bytes read + written).
+double copy_probe(int64_t bytes, int reps, hipStream_t s);
 
 // Batched-form grid state for the lifetime of one pass: clears the scatter
 // arrays, flags every track dirty and points the grid at them; the plain

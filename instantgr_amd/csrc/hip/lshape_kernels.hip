@@ -167,9 +167,10 @@ __global__ void __launch_bounds__(TPB) k_lshape_plain(GridDev g, const int* nets
 
 } // namespace
 
-void launch_lshape_plain(igr_ctx* c, const int* d_nets, const int64_t* d_scr_off, int64_t n, int stamp) {
+void launch_lshape_plain(igr_ctx* c, const int* d_nets, const int64_t* d_scr_off, int64_t n, int stamp,
+                         double bytes) {
     if (n == 0) return;
-    KScope ks(c, K_LSHAPE);
+    KScope ks(c, K_LSHAPE, bytes);
     const unsigned grid = (unsigned)((n + TPB - 1) / TPB);
     NetsDev& N = c->nets;
 #define CASE(LV)                                                                                               \

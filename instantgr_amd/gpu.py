@@ -78,8 +78,10 @@ SIGNATURES = {
     "igr_stamp": (C.c_int, [C.c_void_p]),
     "igr_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "igr_kernel_times": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "igr_kernel_bytes": (C.c_int, [C.c_void_p, C.c_void_p]),
     "igr_run": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RunOpts), C.POINTER(RunStats)]),
     "igr_expf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "igr_copy_probe": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_double)]),
 }
 
 
@@ -116,6 +118,13 @@ def expf(x):
     out = np.empty_like(x)
     _check(lib().igr_expf(x.ctypes.data, out.ctypes.data, x.size))
     return out
+
+
+def copy_probe(nbytes=1 << 30, reps=20):
+    """GB/s (read + write) of a plain device copy of nbytes."""
+    g = C.c_double()
+    _check(lib().igr_copy_probe(int(nbytes), int(reps), C.byref(g)))
+    return g.value
 
 
 class Router:
@@ -251,6 +260,12 @@ class Router:
         ln = (C.c_int64 * len(KCLASSES))()
         _check(lib().igr_kernel_times(self.h, ms, ln))
         return {k: (ms[i], ln[i]) for i, k in enumerate(KCLASSES)}
+
+    def kernel_bytes(self):
+        """Algorithmic HBM bytes per kernel class since profiling was enabled."""
+        b = (C.c_double * len(KCLASSES))()
+        _check(lib().igr_kernel_bytes(self.h, b))
+        return {k: b[i] for i, k in enumerate(KCLASSES)}
 
     def run(self, design, form=FORM_PLAIN, rrr=3, threads=8):
         """Pattern pass plus 'rrr' rip-up-and-reroute passes on a built design."""
