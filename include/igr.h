@@ -80,8 +80,10 @@ int igr_set_nets(igr_ctx* ctx, const igr_net_desc* nets);
 
 /* generate_batches_rsmt_gpu (src/database_cuda.hpp:437-631): order is the
  * routing priority (highest first); batch_of[i] receives the batch of
- * order[i]; *nbatches the batch count. */
-int igr_schedule(igr_ctx* ctx, const int* order, int64_t n, int mode, int* batch_of, int* nbatches);
+ * order[i]; *nbatches the batch count.  Both forms give the same batches
+ * (plain: the reference's rounds with a host check per round; batched:
+ * incremental rounds with device-side control). */
+int igr_schedule(igr_ctx* ctx, const int* order, int64_t n, int mode, int form, int* batch_of, int* nbatches);
 
 /* One pattern-routing batch (src/Lshape_route.hpp:235-242): cost update,
  * prefix sums, L-shape DP of every listed net, wire and via commit. */

@@ -217,12 +217,14 @@ int igr_set_nets(igr_ctx* c, const igr_net_desc* d) {
     });
 }
 
-int igr_schedule(igr_ctx* c, const int* order, int64_t n, int mode, int* batch_of, int* nbatches) {
+int igr_schedule(igr_ctx* c, const int* order, int64_t n, int mode, int form, int* batch_of, int* nbatches) {
     return guard([&]() {
+        if (form != IGR_FORM_PLAIN && form != IGR_FORM_BATCHED) throw std::invalid_argument("unknown form");
         if (n > c->nets.n) throw std::invalid_argument("more nets than uploaded");
         for (int64_t i = 0; i < n; i++)
             if (order[i] < 0 || order[i] >= c->nets.n) throw std::invalid_argument("net id out of range");
-        *nbatches = run_schedule(c, order, n, mode, batch_of);
+        *nbatches = form == IGR_FORM_PLAIN ? run_schedule(c, order, n, mode, batch_of)
+                                           : run_schedule_batched(c, order, n, mode, batch_of);
     });
 }
 
@@ -309,7 +311,8 @@ int igr_pattern_route(igr_ctx* c, const int* order, int64_t n, int form, igr_pas
         reset_pattern_ptrs(c, order, n);
         double t0 = now_s();
         std::vector<int> batch_of(n);
-        int nb = run_schedule(c, order, n, 0, batch_of.data());
+        int nb = form == IGR_FORM_PLAIN ? run_schedule(c, order, n, 0, batch_of.data())
+                                        : run_schedule_batched(c, order, n, 0, batch_of.data());
         double t1 = now_s();
         // Batches in routing order: the scheduler's last batch first; inside a
         // batch, the scheduler's index order (src/Lshape_route.hpp:175-183).
@@ -432,7 +435,8 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
         D.pt_off.upload(dg->pt_off, n + 1, s);
         D.pt.upload(dg->pts, dg->pt_off[n], s);
         std::vector<int> batch_of(n);
-        int nb = run_schedule(c, nets, n, mode, batch_of.data(), D.pt_off.p, D.pt.p);
+        int nb = form == IGR_FORM_PLAIN ? run_schedule(c, nets, n, mode, batch_of.data(), D.pt_off.p, D.pt.p)
+                                        : run_schedule_batched(c, nets, n, mode, batch_of.data(), D.pt_off.p, D.pt.p);
         double t1 = now_s();
 
         // Routing order: batch nb-1 first, inside a batch decreasing list

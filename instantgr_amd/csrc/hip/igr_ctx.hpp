@@ -74,6 +74,9 @@ struct NetsDev {
 
 struct Sched {
     DevBuf<int> map, batch, status, viable, ids, pids, flags;
+    // batched form: per-cell net lists, their heads and the round-0 map
+    DevBuf<int> base, cnt, c_net, committed, ctrl;
+    DevBuf<int64_t> head, c_off, bsum;
 };
 
 // One rip-up-and-reroute pass on the device: DAGs of the rerouted nets in
@@ -200,6 +203,10 @@ void launch_congestion(igr_ctx* c, float* d_xsum, float* d_ysum);
 // of those arrays instead of the pattern footprint.
 int run_schedule(igr_ctx* c, const int* h_order, int64_t n, int mode, int* h_batch_of,
                  const int64_t* d_pt_off = nullptr, const int* d_pt = nullptr);
+// The same schedule computed incrementally with device-side round control
+// (batched form).
+int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, int* h_batch_of,
+                         const int64_t* d_pt_off = nullptr, const int* d_pt = nullptr);
 void launch_rip_wire(igr_ctx* c, const int* d_nets, int64_t n, int stamp);
 // Plain (reference launch structure) device work of one DAG batch, after the
 // rip-up and the cost/prefix update.

@@ -65,7 +65,8 @@ SIGNATURES = {
     "igr_update_cost": (C.c_int, [C.c_void_p]),
     "igr_compute_presum": (C.c_int, [C.c_void_p]),
     "igr_set_nets": (C.c_int, [C.c_void_p, C.POINTER(NetDesc)]),
-    "igr_schedule": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.POINTER(C.c_int)]),
+    "igr_schedule": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_void_p,
+                               C.POINTER(C.c_int)]),
     "igr_route_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int]),
     "igr_pattern_route": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.POINTER(PassStats)]),
     "igr_dag_pass": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.POINTER(DagDesc),
@@ -198,11 +199,12 @@ class Router:
         self.set_nets(a("node_off"), a("node"), a("parent"), a("pt_off"), a("pt"), a("hs_off"), a("hs"),
                       a("vs_off"), a("vs"))
 
-    def schedule(self, order, mode=0):
+    def schedule(self, order, mode=0, form=FORM_PLAIN):
         order = np.ascontiguousarray(order, np.int32)
         out = np.empty(len(order), np.int32)
         nb = C.c_int()
-        _check(lib().igr_schedule(self.h, order.ctypes.data, len(order), mode, out.ctypes.data, C.byref(nb)))
+        _check(lib().igr_schedule(self.h, order.ctypes.data, len(order), mode, form, out.ctypes.data,
+                                  C.byref(nb)))
         return nb.value, out
 
     def route_batch(self, nets, form=FORM_PLAIN):

@@ -70,17 +70,23 @@ def test_presum_track_lengths(shape):
     r.close()
 
 
-def test_schedule_bitexact(design_c1):
+@pytest.mark.parametrize("form", [gpu.FORM_PLAIN, gpu.FORM_BATCHED], ids=["plain", "batched"])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_schedule_bitexact(design_c1, form, mode):
     d = design_c1
     r = gpu.Router.from_design(d)
     o = orc.Oracle.from_design(d)
     r.set_nets_from_design(d)
     o.set_nets_from_design(d)
     order = d.array("order")
-    nb_g, b_g = r.schedule(order)
-    nb_o, b_o = o.schedule(order)
+    nb_g, b_g = r.schedule(order, mode, form)
+    nb_o, b_o = o.schedule(order, mode)
     assert nb_g == nb_o
     assert np.array_equal(b_g, b_o)
+    # a second call on the same context (lists and control rebuilt)
+    nb_2, b_2 = r.schedule(order[: len(order) // 2], mode, form)
+    nb_o2, b_o2 = o.schedule(order[: len(order) // 2], mode)
+    assert nb_2 == nb_o2 and np.array_equal(b_2, b_o2)
     r.close()
 
 
