@@ -260,6 +260,7 @@ def main():
                                                    "rrr_route_s": st["rrr_route_s"],
                                                    "rrr_schedule_s": st["rrr_schedule_s"],
                                                    "rrr_prep_s": st["rrr_prep_s"],
+                                                   "rrr_prep_parts": st["rrr_prep_parts"],
                                                    "pattern_schedule_s": st["pattern_schedule_s"],
                                                    "kernel_ms": {k: round(v[0], 1) for k, v in kt.items() if v[1]}})),
               file=sys.stderr, flush=True)

@@ -137,6 +137,7 @@ typedef struct igr_run_stats {
     int64_t rrr_dag_nodes[IGR_MAX_RRR];
     double rrr_view_s[IGR_MAX_RRR], rrr_dag_build_s[IGR_MAX_RRR], rrr_route_s[IGR_MAX_RRR];
     double rrr_schedule_s[IGR_MAX_RRR], rrr_prep_s[IGR_MAX_RRR];
+    double rrr_prep_parts[IGR_MAX_RRR][4];   /* host fill, batch metadata, upload, allocation */
 } igr_run_stats;
 
 struct igrh_design;

@@ -568,6 +568,7 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
         });
         par_off[NN] = NE;
         csum[NN] = NS;
+        const double t_fill = now_s();
         std::vector<DagBatch> batches;
         std::vector<int64_t> lvl_off(1, 0);
         std::vector<int> lvl_nodes;
@@ -613,6 +614,7 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
             mx_slots = std::max(mx_slots, B.slot1 - B.slot0);
             batches.push_back(B);
         }
+        const double t_meta = now_s();
         D.nets.upload(rnets.data(), rnets.size(), s);
         D.node.upload(node.data(), node.size(), s);
         D.par.upload(par.data(), par.size(), s);
@@ -634,6 +636,8 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
         } else {
             D.lvl_nodes.upload(lvl_nodes.data(), lvl_nodes.size(), s);
         }
+        HIP_OK(hipStreamSynchronize(s));
+        const double t_up = now_s();
         D.costs.alloc(mx_nodes * ML);
         D.lo.alloc(mx_nodes);
         D.emit.alloc(mx_nodes * 9);
@@ -693,6 +697,10 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
         }
         double t2 = now_s();
         c->last_dag_prep_s = tp - t1;
+        c->last_dag_prep_parts[0] = t_fill - t1;
+        c->last_dag_prep_parts[1] = t_meta - t_fill;
+        c->last_dag_prep_parts[2] = t_up - t_meta;
+        c->last_dag_prep_parts[3] = tp - t_up;
         if (st) *st = igr_pass_stats{nb, t1 - t0, t2 - t1, n};
     });
 }

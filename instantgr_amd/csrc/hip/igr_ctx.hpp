@@ -75,7 +75,7 @@ struct NetsDev {
 struct Sched {
     DevBuf<int> map, batch, status, viable, ids, pids, flags;
     // batched form: per-cell net lists, their heads and the round-0 map
-    DevBuf<int> base, cnt, c_net, committed, ctrl;
+    DevBuf<int> base, cnt, c_net, ctrl;
     DevBuf<int64_t> head, c_off, bsum;
 };
 
@@ -160,6 +160,7 @@ struct igr_ctx {
     igr_dev::DagDev dag;
     int stamp = 0;
     double last_dag_prep_s = 0;   // host preparation time of the last DAG pass
+    double last_dag_prep_parts[4] = {0, 0, 0, 0};   // fill, batch metadata, upload, allocation
     int64_t cells() const { return (int64_t)g.L * g.X * g.Y; }
 };
 

@@ -76,6 +76,7 @@ int igr_run(igr_ctx* c, igrh_design* h, const igr_run_opts* o, igr_run_stats* st
         S.rrr_route_s[it] = e - cc;
         S.rrr_schedule_s[it] = ps.schedule_s;
         S.rrr_prep_s[it] = c->last_dag_prep_s;
+        for (int k = 0; k < 4; k++) S.rrr_prep_parts[it][k] = c->last_dag_prep_parts[k];
         S.rrr_batches[it] = ps.batches;
         S.rrr_dag_nodes[it] = D.node_off.empty() ? 0 : D.node_off.back();
         S.iterations = it + 1;

@@ -7,8 +7,8 @@
 //    with its point cells: per cell, the highest-priority unassigned net that
 //    lists the cell.  It is kept across batches from a per-cell list of nets
 //    sorted by priority (built once per pass) and a head index per cell that
-//    skips assigned nets; only the point cells of the nets a batch committed
-//    change, so round 0 claims nothing.
+//    moves past assigned nets at the end of each batch, so round 0 claims
+//    nothing.
 //  * In later rounds a point claim can only change a cell the previous
 //    round's clear emptied (values otherwise only grow within a batch), so a
 //    net claims a point only where the map holds a lower value.
@@ -27,7 +27,8 @@ constexpr int TPB = 256;
 constexpr int ROUNDS = 30;        // rounds per batch (src/database_cuda.hpp:612)
 constexpr int CHUNK = 16;         // rounds enqueued per host check
 
-enum { C_DONE = 0, C_CUR, C_ROUND, C_COMMIT, C_REVERT, C_END, C_HAVE, C_ASSIGNED, C_NCOMMITTED, C_WORDS };
+// control state words (two slots of C_WORDS, then the placed-net counter)
+enum { C_DONE = 0, C_CUR, C_ROUND, C_COMMIT, C_HAVE, C_NVIABLE, C_WORDS };
 
 struct BArgs {
     int64_t n;
@@ -46,8 +47,6 @@ struct BArgs {
     int64_t* head;                // X*Y first unassigned entry of each list
     int* batch;
     int* status;                  // 1 active in this batch, 2 blocked, 0 assigned earlier
-    int* viable;
-    int* committed;               // nets committed in the current batch
     int* ctrl;
     int X, Y, sparsity;
 };
@@ -168,141 +167,183 @@ __global__ void k_scan_apply(const int* cnt, int64_t n, const int64_t* bsum, int
 }
 
 // ---- rounds -----------------------------------------------------------------
-__device__ __forceinline__ void claim(int* m, int i) {
-    if (*m < i) atomicMax(m, i);
+// Round g reads the control state in slot g & 1; its last kernel writes the
+// successor state into the other slot, so no kernel reads a word another
+// kernel of the same round rewrites.  A net's point list is walked by a group
+// of G lanes (one step covers G points), with wave ballots for the early
+// exits; a viable net's neighbour and segment claims are spread over a whole
+// wave.
+__device__ __forceinline__ uint64_t group_bits(uint64_t m, int G) {
+    const int base = (int)(threadIdx.x & 63) & ~(G - 1);
+    const uint64_t mask = G == 64 ? ~0ull : ((1ull << G) - 1);
+    return (m >> base) & mask;
 }
 
-__device__ __forceinline__ bool owns_points(const BArgs& a, int64_t i) {
-    for (int64_t k = a.pt_off[a.pids[i]]; k < a.pt_off[a.pids[i] + 1]; k++)
-        if (a.map[a.pt[k]] != (int)i) return false;
-    return true;
-}
-
-// 1. point claims (round 0 starts from the kept map)
-__global__ void __launch_bounds__(TPB) k_b_claim(BArgs a) {
-    if (a.ctrl[C_DONE] || a.ctrl[C_ROUND] == 0) return;
-    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (i >= a.n || !active(a, i)) return;
-    for (int64_t k = a.pt_off[a.pids[i]]; k < a.pt_off[a.pids[i] + 1]; k++) claim(&a.map[a.pt[k]], (int)i);
-}
-
-// 2. nets owning every point cell
-__global__ void __launch_bounds__(TPB) k_b_viable(BArgs a) {
-    if (a.ctrl[C_DONE]) return;
-    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (i >= a.n) return;
-    a.viable[i] = active(a, i) && owns_points(a, i);
-}
-
-// 3. their neighbour cells and sampled segment cells
-__global__ void __launch_bounds__(TPB) k_b_claim_rest(BArgs a) {
-    if (a.ctrl[C_DONE]) return;
-    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (i >= a.n || !a.viable[i]) return;
-    const int net = a.ids[i];
-    const int X = a.X, Y = a.Y;
-    for (int64_t k = a.pt_off[a.pids[i]]; k < a.pt_off[a.pids[i] + 1]; k++) {
-        const int p = a.pt[k], x = p / Y, y = p % Y;
-        if (y - 1 >= 0) claim(&a.map[x * Y + y - 1], (int)i);
-        if (x - 1 >= 0) claim(&a.map[(x - 1) * Y + y], (int)i);
-        if (x + 1 < X) claim(&a.map[(x + 1) * Y + y], (int)i);
-    }
-    for (int64_t k = a.hs_off[net]; k < a.hs_off[net + 1]; k++) {
-        const int s = a.hs[2 * k], e = a.hs[2 * k + 1], y = s % Y;
-        for (int x = s / Y; x <= e / Y; x += a.sparsity) claim(&a.map[x * Y + y], (int)i);
-    }
-    for (int64_t k = a.vs_off[net]; k < a.vs_off[net + 1]; k++) {
-        const int s = a.vs[2 * k], e = a.vs[2 * k + 1], x = s / Y;
-        for (int y = s % Y; y <= e % Y; y += a.sparsity) claim(&a.map[x * Y + y], (int)i);
-    }
-}
-
-// 4. commit the viable nets still owning every point cell
-__global__ void __launch_bounds__(TPB) k_b_commit(BArgs a) {
-    if (a.ctrl[C_DONE]) return;
-    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (i >= a.n || !a.viable[i] || !owns_points(a, i)) return;
-    a.batch[i] = a.ctrl[C_CUR];
-    a.committed[atomicAdd(&a.ctrl[C_NCOMMITTED], 1)] = (int)i;
-    atomicAdd(&a.ctrl[C_ASSIGNED], 1);
-    a.ctrl[C_COMMIT] = 1;
-}
-
-// Single thread: end of the round's commit.
-__global__ void k_b_control(int* ctrl) {
-    if (ctrl[C_DONE]) return;
-    const int round = ctrl[C_ROUND];
-    if (ctrl[C_COMMIT]) {
-        ctrl[C_COMMIT] = 0;
-        ctrl[C_HAVE] = 1;
-        ctrl[C_REVERT] = 1;
-        ctrl[C_END] = round + 1 >= ROUNDS ? 1 : 0;
-        ctrl[C_ROUND] = round + 1;
-    } else {
-        ctrl[C_REVERT] = 0;
-        ctrl[C_END] = 1;
-    }
-}
-
-// 5. nets with a point cell owned by a committed net sit out the batch
-__global__ void __launch_bounds__(TPB) k_b_revert(BArgs a) {
-    if (a.ctrl[C_DONE] || !a.ctrl[C_REVERT]) return;
-    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (i >= a.n || !active(a, i)) return;
-    for (int64_t k = a.pt_off[a.pids[i]]; k < a.pt_off[a.pids[i] + 1]; k++) {
-        const int pre = a.map[a.pt[k]];
-        if (pre != -1 && a.batch[pre] != -1) { a.status[i] = 2; return; }
-    }
-}
-
-// 6. their cells are released
-__global__ void __launch_bounds__(TPB) k_b_clear(BArgs a, int64_t cells) {
-    if (a.ctrl[C_DONE] || !a.ctrl[C_REVERT]) return;
-    const int64_t q = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (q >= cells) return;
-    const int m = a.map[q];
-    if (m >= 0 && a.status[m] == 2) a.map[q] = -1;
-}
-
-// 7. end of a batch: the lists of the committed nets' point cells (disjoint
-//    between committed nets) move to their first unassigned entry
-__global__ void __launch_bounds__(TPB) k_b_base(BArgs a) {
-    if (a.ctrl[C_DONE] || !a.ctrl[C_END]) return;
-    const int nc = a.ctrl[C_NCOMMITTED];
-    for (int t = blockIdx.x * TPB + threadIdx.x; t < nc; t += gridDim.x * TPB) {
-        const int i = a.committed[t];
-        for (int64_t k = a.pt_off[a.pids[i]]; k < a.pt_off[a.pids[i] + 1]; k++) {
-            const int p = a.pt[k];
-            int64_t h = a.head[p];
-            const int64_t e = a.c_off[p + 1];
-            while (h < e && a.batch[a.c_net[h]] != -1) h++;
-            a.head[p] = h;
-            a.base[p] = h < e ? a.c_net[h] : -1;
+// 1. point claims of the active nets (from round 1; round 0 starts from the
+//    kept map): only cells holding a lower value can change
+template <int G>
+__global__ void __launch_bounds__(TPB) k_b_claim(BArgs a, int64_t g) {
+    const int* st = a.ctrl + (g & 1) * C_WORDS;
+    if (st[C_DONE] || st[C_ROUND] == 0) return;
+    const int64_t nth = (int64_t)gridDim.x * TPB / G;
+    const int lg = threadIdx.x & (G - 1);
+    for (int64_t i = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G; i < a.n; i += nth) {
+        if (!active(a, i)) continue;
+        const int64_t k1 = a.pt_off[a.pids[i] + 1];
+        for (int64_t k = a.pt_off[a.pids[i]] + lg; k < k1; k += G) {
+            int* m = &a.map[a.pt[k]];
+            if (*m < (int)i) atomicMax(m, (int)i);
         }
     }
 }
 
-// 8. the next batch starts from the kept map with every unassigned net
-__global__ void __launch_bounds__(TPB) k_b_next(BArgs a, int64_t cells) {
-    if (a.ctrl[C_DONE] || !a.ctrl[C_END]) return;
-    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (t < cells) a.map[t] = a.base[t];
-    if (t < a.n) a.status[t] = a.batch[t] == -1 ? 1 : 0;
+// 2. active nets owning every point cell go to the viable list
+template <int G>
+__global__ void __launch_bounds__(TPB) k_b_viable(BArgs a, int64_t g, int* vlist) {
+    int* st = a.ctrl + (g & 1) * C_WORDS;
+    if (st[C_DONE]) return;
+    const int64_t nth = (int64_t)gridDim.x * TPB / G;
+    const int lg = threadIdx.x & (G - 1);
+    const int64_t rounds = (a.n + nth - 1) / nth;   // the same for every lane of the wave
+    for (int64_t r = 0; r < rounds; r++) {
+        const int64_t i = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G + r * nth;
+        const bool live = i < a.n && active(a, i);
+        const int64_t k0 = live ? a.pt_off[a.pids[i]] : 0, k1 = live ? a.pt_off[a.pids[i] + 1] : 0;
+        bool fail = !live;
+        for (int64_t k = k0 + lg;; k += G) {
+            const bool f = !fail && k < k1 && a.map[a.pt[k]] != (int)i;
+            fail = fail || group_bits(__ballot(f), G) != 0;
+            if (__ballot(!fail && k + G < k1) == 0) break;
+        }
+        if (lg == 0 && !fail) vlist[atomicAdd(&st[C_NVIABLE], 1)] = (int)i;
+    }
 }
 
-// Single thread: start the next batch, or finish.
-__global__ void k_b_advance(int* ctrl, int64_t n) {
-    if (ctrl[C_DONE] || !ctrl[C_END]) return;
-    if (!ctrl[C_HAVE]) {
-        ctrl[C_DONE] = 1;     // a batch without commits: nothing left to place
-        return;
+// 3. a wave per viable net: neighbour cells (0,-1), (-1,0), (1,0) of its
+//    points and every sparsity-th cell of its segments
+__global__ void __launch_bounds__(TPB) k_b_claim_rest(BArgs a, int64_t g, const int* vlist) {
+    const int* st = a.ctrl + (g & 1) * C_WORDS;
+    if (st[C_DONE]) return;
+    const int nv = st[C_NVIABLE];
+    const int lane = threadIdx.x & 63;
+    const int X = a.X, Y = a.Y;
+    for (int w = (blockIdx.x * TPB + threadIdx.x) >> 6; w < nv; w += (gridDim.x * TPB) >> 6) {
+        const int i = vlist[w];
+        const int net = a.ids[i];
+        for (int64_t k = a.pt_off[a.pids[i]] + lane; k < a.pt_off[a.pids[i] + 1]; k += 64) {
+            const int p = a.pt[k], x = p / Y, y = p % Y;
+            if (y - 1 >= 0) atomicMax(&a.map[x * Y + y - 1], i);
+            if (x - 1 >= 0) atomicMax(&a.map[(x - 1) * Y + y], i);
+            if (x + 1 < X) atomicMax(&a.map[(x + 1) * Y + y], i);
+        }
+        for (int64_t k = a.hs_off[net] + lane; k < a.hs_off[net + 1]; k += 64) {
+            const int s0 = a.hs[2 * k], e = a.hs[2 * k + 1], y = s0 % Y;
+            for (int x = s0 / Y; x <= e / Y; x += a.sparsity) atomicMax(&a.map[x * Y + y], i);
+        }
+        for (int64_t k = a.vs_off[net] + lane; k < a.vs_off[net + 1]; k += 64) {
+            const int s0 = a.vs[2 * k], e = a.vs[2 * k + 1], x = s0 / Y;
+            for (int y = s0 % Y; y <= e % Y; y += a.sparsity) atomicMax(&a.map[x * Y + y], i);
+        }
     }
-    ctrl[C_CUR] += 1;
-    ctrl[C_ROUND] = 0;
-    ctrl[C_END] = 0;
-    ctrl[C_HAVE] = 0;
-    ctrl[C_NCOMMITTED] = 0;
-    if ((int64_t)ctrl[C_ASSIGNED] == n) ctrl[C_DONE] = 1;
+}
+
+// 4. a wave per viable net: commit it if it still owns every point cell
+__global__ void __launch_bounds__(TPB) k_b_commit(BArgs a, int64_t g, const int* vlist) {
+    int* st = a.ctrl + (g & 1) * C_WORDS;
+    if (st[C_DONE]) return;
+    const int nv = st[C_NVIABLE];
+    const int lane = threadIdx.x & 63;
+    for (int w = (blockIdx.x * TPB + threadIdx.x) >> 6; w < nv; w += (gridDim.x * TPB) >> 6) {
+        const int i = vlist[w];
+        bool lost = false;
+        for (int64_t k = a.pt_off[a.pids[i]] + lane; k < a.pt_off[a.pids[i] + 1]; k += 64)
+            lost = lost || a.map[a.pt[k]] != i;
+        if (__ballot(lost) == 0 && lane == 0) {
+            a.batch[i] = st[C_CUR];
+            atomicAdd(&a.ctrl[2 * C_WORDS], 1);   // nets placed so far
+            st[C_COMMIT] = 1;
+        }
+    }
+}
+
+// 5. after a commit: active nets with a point cell owned by a committed net
+//    sit out the batch
+template <int G>
+__global__ void __launch_bounds__(TPB) k_b_revert(BArgs a, int64_t g) {
+    const int* st = a.ctrl + (g & 1) * C_WORDS;
+    if (st[C_DONE] || !st[C_COMMIT]) return;
+    const int64_t nth = (int64_t)gridDim.x * TPB / G;
+    const int lg = threadIdx.x & (G - 1);
+    const int64_t rounds = (a.n + nth - 1) / nth;
+    for (int64_t r = 0; r < rounds; r++) {
+        const int64_t i = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G + r * nth;
+        const bool live = i < a.n && active(a, i);
+        const int64_t k0 = live ? a.pt_off[a.pids[i]] : 0, k1 = live ? a.pt_off[a.pids[i] + 1] : 0;
+        bool hit = false;
+        for (int64_t k = k0 + lg;; k += G) {
+            bool h = false;
+            if (live && !hit && k < k1) {
+                const int pre = a.map[a.pt[k]];
+                h = pre != -1 && a.batch[pre] != -1;
+            }
+            hit = hit || group_bits(__ballot(h), G) != 0;
+            if (__ballot(live && !hit && k + G < k1) == 0) break;
+        }
+        if (lg == 0 && live && hit) a.status[i] = 2;
+    }
+}
+
+// 6. end of the round.  The batch goes on after a commit below the round
+//    limit: cells of blocked nets are released.  Otherwise the batch ends:
+//    each cell's list moves past assigned nets, the next batch starts from
+//    the kept map with every unassigned net active.  Thread 0 writes the
+//    successor control state.
+__global__ void __launch_bounds__(TPB) k_b_clear_end(BArgs a, int64_t g, int64_t cells) {
+    const int* st = a.ctrl + (g & 1) * C_WORDS;
+    int* nx = a.ctrl + ((g + 1) & 1) * C_WORDS;
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    const bool done = st[C_DONE] != 0;
+    const bool commit = st[C_COMMIT] != 0;
+    const int round = st[C_ROUND];
+    const bool end = !commit || round + 1 >= ROUNDS;
+    if (!done) {
+        if (end) {
+            if (t < cells) {
+                int64_t h = a.head[t];
+                const int64_t e = a.c_off[t + 1];
+                while (h < e && a.batch[a.c_net[h]] != -1) h++;
+                a.head[t] = h;
+                const int b = h < e ? a.c_net[h] : -1;
+                a.base[t] = b;
+                a.map[t] = b;
+            }
+            if (t < a.n) a.status[t] = a.batch[t] == -1 ? 1 : 0;
+        } else if (t < cells) {
+            const int m = a.map[t];
+            if (m >= 0 && a.status[m] == 2) a.map[t] = -1;
+        }
+    }
+    if (t == 0) {
+        int d = st[C_DONE], cur = st[C_CUR], r = st[C_ROUND], have = st[C_HAVE] | (commit ? 1 : 0);
+        if (!d) {
+            if (!end) {
+                r = round + 1;
+            } else if (!have) {
+                d = 1;                       // a batch without commits: nothing left to place
+            } else {
+                cur += 1;
+                r = 0;
+                have = 0;
+                if ((int64_t)a.ctrl[2 * C_WORDS] == a.n) d = 1;
+            }
+        }
+        nx[C_DONE] = d;
+        nx[C_CUR] = cur;
+        nx[C_ROUND] = r;
+        nx[C_HAVE] = have;
+        nx[C_COMMIT] = 0;
+        nx[C_NVIABLE] = 0;
+    }
 }
 
 __global__ void k_fill_b(int* p, int64_t n, int v) {
@@ -339,11 +380,10 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
     S.batch.alloc(n);
     S.status.alloc(n);
     S.viable.alloc(n);
-    S.committed.alloc(n);
-    S.ctrl.zero(C_WORDS, st);
+    S.ctrl.zero(2 * C_WORDS + 1, st);
     BArgs a{n, S.ids.p, d_pt_off ? S.pids.p : S.ids.p, d_pt_off ? d_pt_off : N.pt_off.p, d_pt_off ? d_pt : N.pt.p,
             N.hs_off.p, N.hs.p, N.vs_off.p, N.vs.p, S.map.p, S.base.p, S.c_off.p, nullptr, S.head.p,
-            S.batch.p, S.status.p, S.viable.p, S.committed.p, S.ctrl.p, c->g.X, c->g.Y,
+            S.batch.p, S.status.p, S.ctrl.p, c->g.X, c->g.Y,
             (mode == 1 || mode == 2) ? 2 : 15};
     // Per-cell lists of the nets' point cells, highest priority first.
     k_cell_count<<<blocks(n), TPB, 0, st>>>(a, S.cnt.p);
@@ -364,35 +404,45 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
     k_fill_b<<<blocks(n), TPB, 0, st>>>(S.batch.p, n, -1);
     k_fill_b<<<blocks(n), TPB, 0, st>>>(S.status.p, n, 1);
     HIP_OK(hipGetLastError());
+    // Lanes per net: about half the average point count, at most 16.
+    const int64_t entries_total = entries;
+    const double avg = (double)entries_total / (double)n;
+    const int G = avg <= 6 ? 4 : avg <= 16 ? 8 : 16;
+    const unsigned ggrid = (unsigned)std::min<int64_t>(2048, blocks(n * G));
+    const unsigned wgrid = 256;   // one wave per viable net
+    const unsigned gm = blocks(std::max(XYc, n));
     int* h_ctrl = nullptr;
-    HIP_OK(hipHostMalloc(&h_ctrl, sizeof(int) * C_WORDS));
+    HIP_OK(hipHostMalloc(&h_ctrl, sizeof(int) * (2 * C_WORDS + 1)));
     const int64_t max_rounds = (int64_t)ROUNDS * (n + 1) + CHUNK;
-    const unsigned gn = blocks(n), gc = blocks(XYc), gm = blocks(std::max(XYc, n));
+    int64_t g = 0;
     try {
-        for (int64_t rounds = 0;; rounds += CHUNK) {
-            if (rounds > max_rounds) throw std::runtime_error("scheduler did not finish");
-            for (int r = 0; r < CHUNK; r++) {
-                k_b_claim<<<gn, TPB, 0, st>>>(a);
-                k_b_viable<<<gn, TPB, 0, st>>>(a);
-                k_b_claim_rest<<<gn, TPB, 0, st>>>(a);
-                k_b_commit<<<gn, TPB, 0, st>>>(a);
-                k_b_control<<<1, 1, 0, st>>>(S.ctrl.p);
-                k_b_revert<<<gn, TPB, 0, st>>>(a);
-                k_b_clear<<<gc, TPB, 0, st>>>(a, XYc);
-                k_b_base<<<256, TPB, 0, st>>>(a);
-                k_b_next<<<gm, TPB, 0, st>>>(a, XYc);
-                k_b_advance<<<1, 1, 0, st>>>(S.ctrl.p, n);
+        for (;;) {
+            if (g > max_rounds) throw std::runtime_error("scheduler did not finish");
+            for (int r = 0; r < CHUNK; r++, g++) {
+                switch (G) {
+                case 4: k_b_claim<4><<<ggrid, TPB, 0, st>>>(a, g); k_b_viable<4><<<ggrid, TPB, 0, st>>>(a, g, S.viable.p); break;
+                case 8: k_b_claim<8><<<ggrid, TPB, 0, st>>>(a, g); k_b_viable<8><<<ggrid, TPB, 0, st>>>(a, g, S.viable.p); break;
+                default: k_b_claim<16><<<ggrid, TPB, 0, st>>>(a, g); k_b_viable<16><<<ggrid, TPB, 0, st>>>(a, g, S.viable.p); break;
+                }
+                k_b_claim_rest<<<wgrid, TPB, 0, st>>>(a, g, S.viable.p);
+                k_b_commit<<<wgrid, TPB, 0, st>>>(a, g, S.viable.p);
+                switch (G) {
+                case 4: k_b_revert<4><<<ggrid, TPB, 0, st>>>(a, g); break;
+                case 8: k_b_revert<8><<<ggrid, TPB, 0, st>>>(a, g); break;
+                default: k_b_revert<16><<<ggrid, TPB, 0, st>>>(a, g); break;
+                }
+                k_b_clear_end<<<gm, TPB, 0, st>>>(a, g, XYc);
             }
             HIP_OK(hipGetLastError());
-            HIP_OK(hipMemcpyAsync(h_ctrl, S.ctrl.p, sizeof(int) * C_WORDS, hipMemcpyDeviceToHost, st));
+            HIP_OK(hipMemcpyAsync(h_ctrl, S.ctrl.p, sizeof(int) * (2 * C_WORDS + 1), hipMemcpyDeviceToHost, st));
             HIP_OK(hipStreamSynchronize(st));
-            if (h_ctrl[C_DONE]) break;
+            if (h_ctrl[(g & 1) * C_WORDS + C_DONE]) break;
         }
     } catch (...) {
         (void)hipHostFree(h_ctrl);
         throw;
     }
-    const int nbatch = h_ctrl[C_CUR];
+    const int nbatch = h_ctrl[(g & 1) * C_WORDS + C_CUR];
     HIP_OK(hipHostFree(h_ctrl));
     std::vector<int> b(n);
     HIP_OK(hipMemcpyAsync(b.data(), S.batch.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));

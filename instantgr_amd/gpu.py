@@ -50,7 +50,8 @@ class RunStats(C.Structure):
                 ("subnets", C.c_int64), ("ripped", C.c_int64 * MAX_RRR), ("rrr_batches", C.c_int * MAX_RRR),
                 ("rrr_dag_nodes", C.c_int64 * MAX_RRR), ("rrr_view_s", C.c_double * MAX_RRR),
                 ("rrr_dag_build_s", C.c_double * MAX_RRR), ("rrr_route_s", C.c_double * MAX_RRR),
-                ("rrr_schedule_s", C.c_double * MAX_RRR), ("rrr_prep_s", C.c_double * MAX_RRR)]
+                ("rrr_schedule_s", C.c_double * MAX_RRR), ("rrr_prep_s", C.c_double * MAX_RRR),
+                ("rrr_prep_parts", (C.c_double * 4) * MAX_RRR)]
 
 
 # Every entry point of include/igr.h with its ctypes signature.
@@ -281,7 +282,8 @@ class Router:
                 "subnets": st.subnets, "ripped": list(st.ripped[:it]), "rrr_batches": list(st.rrr_batches[:it]),
                 "rrr_dag_nodes": list(st.rrr_dag_nodes[:it]), "rrr_view_s": list(st.rrr_view_s[:it]),
                 "rrr_dag_build_s": list(st.rrr_dag_build_s[:it]), "rrr_route_s": list(st.rrr_route_s[:it]),
-                "rrr_schedule_s": list(st.rrr_schedule_s[:it]), "rrr_prep_s": list(st.rrr_prep_s[:it])}
+                "rrr_schedule_s": list(st.rrr_schedule_s[:it]), "rrr_prep_s": list(st.rrr_prep_s[:it]),
+                "rrr_prep_parts": [list(st.rrr_prep_parts[k]) for k in range(it)]}
 
     @property
     def stamp(self):
