@@ -3,6 +3,7 @@
 #include "igr_ctx.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <exception>
@@ -51,6 +52,7 @@ int igr_create(const igr_grid_desc* d, igr_ctx** out) {
         if (ndev < 1) throw std::runtime_error("no GPU visible");
         HIP_OK(hipGetDevice(&c->device));
         HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIP_OK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
         GridDev& g = c->g;
         g.L = d->L; g.X = d->X; g.Y = d->Y; g.DIR = d->DIR & 1;
         g.XYc = (int64_t)d->X * d->Y;
@@ -88,9 +90,11 @@ int igr_create(const igr_grid_desc* d, igr_ctx** out) {
 void igr_destroy(igr_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    hipStream_t s = c->stream;
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+    hipStream_t s = c->stream, cs = c->copy_stream;
     delete c;
     if (s) (void)hipStreamDestroy(s);
+    if (cs) (void)hipStreamDestroy(cs);
 }
 
 static void to_host_ref(igr_ctx* c, const void* dev, void* host, int eb) {
@@ -400,6 +404,263 @@ int igr_get_routes(igr_ctx* c, int64_t* off, int* data, int64_t cap) {
     });
 }
 
+namespace {
+
+void pfor(int64_t cnt, const std::function<void(int64_t, int64_t)>& fn) {
+    const int nthreads = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    const int64_t chunk = (cnt + nthreads - 1) / nthreads;
+    for (int t = 0; t < nthreads; t++) {
+        const int64_t a0 = t * chunk, a1 = std::min(cnt, a0 + chunk);
+        if (a0 < a1) th.emplace_back(fn, a0, a1);
+    }
+    for (auto& x : th) x.join();
+}
+
+// Batched form of one rip-up-and-reroute pass.  While the scheduler runs on
+// the compute stream, a helper thread uploads the DAGs as built (copy stream)
+// and measures every net (nodes, edges, slots, levels, algorithmic bytes).
+// Once the routing order is known the host only computes per-net bases and
+// batch bounds; the device lays the DAGs out (k_dag_layout) and checks them.
+void dag_pass_batched(igr_ctx* c, const int* nets, int64_t n, int mode, const igr_dag_desc* dg, igr_pass_stats* st) {
+    const int XY = c->g.X * c->g.Y, L = c->g.L, LL = L * L, ML = L + 1;
+    const int64_t NN = dg->node_off[n], NE = dg->par_off[NN];
+    // The scheduler's points are checked here, the DAG structure on the device.
+    std::atomic<int> bad_pt{0};
+    pfor(n, [&](int64_t a0, int64_t a1) {
+        int bad = 0;
+        for (int64_t i = a0; i < a1; i++) {
+            bad |= dg->node_off[i + 1] <= dg->node_off[i];
+            for (int64_t q = dg->pt_off[i]; q < dg->pt_off[i + 1]; q++) bad |= dg->pts[q] < 0 || dg->pts[q] >= XY;
+        }
+        if (bad) bad_pt.store(1);
+    });
+    if (bad_pt.load()) throw std::invalid_argument("empty DAG or DAG point outside the grid");
+    DagDev& D = c->dag;
+    hipStream_t s = c->stream;
+    const double t0 = now_s();
+    D.pt_off.upload(dg->pt_off, n + 1, s);
+    D.pt.upload(dg->pts, dg->pt_off[n], s);
+
+    double pairs[2] = {0, 0}, pvia[2] = {0, 0};
+    for (int fl = 0; fl < L; fl++)
+        for (int tl = fl & 1; tl < L; tl += 2) {
+            pairs[layer_dir(fl, c->g.DIR)] += 1;
+            pvia[layer_dir(fl, c->g.DIR)] += std::abs(fl - tl);
+        }
+    if (dg->node_off[0] != 0 || dg->par_off[0] != 0) throw std::invalid_argument("DAG offsets must start at 0");
+    std::vector<int64_t> l_ns(n);
+    std::vector<int> l_nl(n);
+    std::vector<double> l_eb(n), l_db(n);
+    std::atomic<int> bad_dag{0};
+    std::exception_ptr herr;
+    double t_helper = 0;
+    std::thread helper([&]() {
+        try {
+            const double h0 = now_s();
+            hipStream_t cs = c->copy_stream;
+            D.r_node_off.upload(dg->node_off, n + 1, cs);
+            D.r_nodes.upload(dg->nodes, NN, cs);
+            D.r_par_off.upload(dg->par_off, NN + 1, cs);
+            D.r_par.upload(dg->par, NE, cs);
+            D.r_slot.upload(dg->slot, NE, cs);
+            D.r_child_num.upload(dg->child_num, NN, cs);
+            D.r_depth.upload(dg->depth, NN, cs);
+            // Per net (list order): slots, levels and the algorithmic bytes
+            // of the DAG kernels (the models are stated in the plain path).
+            pfor(n, [&](int64_t a0, int64_t a1) {
+                for (int64_t i = a0; i < a1; i++) {
+                    const int64_t b0 = dg->node_off[i], b1 = dg->node_off[i + 1];
+                    int64_t slots = 0;
+                    int maxd = 0;
+                    double eb = 0, longp = 0;
+                    for (int64_t k = b0; k < b1; k++) {
+                        // the layout kernel trusts the edge offsets and depths
+                        if (dg->par_off[k + 1] < dg->par_off[k] || dg->par_off[k + 1] > NE || dg->depth[k] < 0 ||
+                            dg->depth[k] >= b1 - b0 || dg->child_num[k] < 0 || dg->child_num[k] > 10) {
+                            bad_dag.store(1);
+                            break;
+                        }
+                        slots += dg->child_num[k];
+                        maxd = std::max(maxd, dg->depth[k]);
+                        const int v = dg->nodes[k];
+                        for (int64_t q = dg->par_off[k]; q < dg->par_off[k + 1]; q++) {
+                            const int64_t pl = dg->par[q];
+                            if (pl < 0 || pl >= b1 - b0) continue;   // rejected by the layout
+                            const int u = dg->nodes[b0 + pl];
+                            const int vx = v % XY / c->g.Y, vy = v % c->g.Y, ux = u % XY / c->g.Y, uy = u % c->g.Y;
+                            const int dist = std::abs(vx - ux) + std::abs(vy - uy);
+                            if (dist <= 4 || (vx != ux && vy != uy)) continue;
+                            const int dir = vx == ux ? 1 : 0;
+                            const int dd = dist / std::min(dist, 20);
+                            eb += ((dist - 1) / dd) * (16 * pairs[dir] + 4 * pvia[dir]) + 12 * pairs[dir];
+                            longp += pairs[dir];
+                        }
+                    }
+                    const int64_t nn = b1 - b0, ne = dg->par_off[b1] - dg->par_off[b0];
+                    l_ns[i] = slots;
+                    l_nl[i] = maxd + 1;
+                    l_eb[i] = eb;
+                    l_db[i] = nn * (4.0 + 4 * L + 16 * ML + 32) + ne * (8.0 + 8 * L + 8 * ML) + longp * 8 +
+                              slots * 16.0 * ML;
+                }
+            });
+            HIP_OK(hipStreamSynchronize(cs));
+            t_helper = now_s() - h0;
+        } catch (...) {
+            herr = std::current_exception();
+        }
+    });
+    std::vector<int> batch_of(n);
+    int nb = 0;
+    try {
+        nb = run_schedule_batched(c, nets, n, mode, batch_of.data(), D.pt_off.p, D.pt.p);
+    } catch (...) {
+        helper.join();
+        throw;
+    }
+    helper.join();
+    if (herr) std::rethrow_exception(herr);
+    if (bad_dag.load()) throw std::invalid_argument("malformed DAG (edge offsets, depths or slot counts)");
+    const double t1 = now_s();
+
+    // Routing order: batch nb-1 first, inside a batch decreasing list
+    // position (the scheduler's index order).
+    std::vector<int64_t> cntb(nb + 1, 0);
+    for (int64_t i = 0; i < n; i++) cntb[nb - 1 - batch_of[i] + 1]++;
+    for (int b = 0; b < nb; b++) cntb[b + 1] += cntb[b];
+    std::vector<int> rorder(n), rnets(n);
+    {
+        std::vector<int64_t> pos(cntb.begin(), cntb.end() - 1);
+        for (int64_t i = n - 1; i >= 0; i--) rorder[pos[nb - 1 - batch_of[i]]++] = (int)i;
+    }
+    std::vector<int64_t> nbase(n + 1, 0), ebase(n + 1, 0), sbase(n + 1, 0), lbase(n + 1, 0), route_slot(n + 1, 0);
+    for (int64_t r = 0; r < n; r++) {
+        const int64_t i = rorder[r];
+        const int64_t nn = dg->node_off[i + 1] - dg->node_off[i];
+        rnets[r] = nets[i];
+        nbase[r + 1] = nbase[r] + nn;
+        ebase[r + 1] = ebase[r] + (dg->par_off[dg->node_off[i + 1]] - dg->par_off[dg->node_off[i]]);
+        sbase[r + 1] = sbase[r] + l_ns[i];
+        lbase[r + 1] = lbase[r] + l_nl[i];
+        route_slot[r + 1] = route_slot[r] + 8 * nn + 1;
+    }
+    const int64_t NS = sbase[n], NL = lbase[n];
+    std::vector<DagBatch> batches(nb);
+    int64_t mx_nodes = 0, mx_edges = 0, mx_slots = 0;
+    for (int b = 0; b < nb; b++) {
+        DagBatch& B = batches[b];
+        B.net0 = cntb[b];
+        B.net1 = cntb[b + 1];
+        B.node0 = nbase[B.net0];
+        B.node1 = nbase[B.net1];
+        B.edge0 = ebase[B.net0];
+        B.edge1 = ebase[B.net1];
+        B.slot0 = sbase[B.net0];
+        B.slot1 = sbase[B.net1];
+        int maxl = 1;
+        for (int64_t r = B.net0; r < B.net1; r++) {
+            maxl = std::max(maxl, l_nl[rorder[r]]);
+            B.bytes_edges += l_eb[rorder[r]];
+            B.bytes_dp += l_db[rorder[r]];
+        }
+        B.levels = maxl;
+        B.lvl0 = 0;
+        mx_nodes = std::max(mx_nodes, B.node1 - B.node0);
+        mx_edges = std::max(mx_edges, B.edge1 - B.edge0);
+        mx_slots = std::max(mx_slots, B.slot1 - B.slot0);
+    }
+    D.nets.upload(rnets.data(), n, s);
+    D.rorder.upload(rorder.data(), n, s);
+    D.node_off.upload(nbase.data(), n + 1, s);
+    D.ebase.upload(ebase.data(), n + 1, s);
+    D.sbase.upload(sbase.data(), n + 1, s);
+    D.lbase.upload(lbase.data(), n + 1, s);
+    D.route_slot.upload(route_slot.data(), n + 1, s);
+    D.node.alloc(NN);
+    D.child_num.alloc(NN);
+    D.loc.alloc(NN);
+    D.csum.alloc(NN + 1);
+    D.par_off.alloc(NN + 1);
+    D.par.alloc(NE);
+    D.slot.alloc(NE);
+    D.enode.alloc(NE);
+    D.nl_off.alloc(n);
+    D.nlevels.alloc(n);
+    D.lvl_node_off.alloc(NL + n);
+    D.lvl_edge_off.alloc(NL + n);
+    D.cur_n.alloc(NL + n);
+    D.cur_e.alloc(NL + n);
+    D.dnode.alloc(NN);
+    D.dedge.alloc(NE);
+    D.err.zero(1, s);
+    const int64_t tails[2] = {NE, NS};
+    HIP_OK(hipMemcpyAsync(D.par_off.p + NN, &tails[0], sizeof(int64_t), hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(D.csum.p + NN, &tails[1], sizeof(int64_t), hipMemcpyHostToDevice, s));
+    const double t_meta = now_s();
+    DagLayoutArgs la{D.r_node_off.p, D.r_nodes.p, D.r_par_off.p, D.r_par.p, D.r_slot.p, D.r_child_num.p, D.r_depth.p,
+                     D.rorder.p, D.node_off.p, D.ebase.p, D.sbase.p, D.lbase.p, (L + 1) * XY,
+                     D.node.p, D.child_num.p, D.loc.p, D.csum.p, D.par_off.p, D.par.p, D.slot.p, D.enode.p,
+                     D.nl_off.p, D.nlevels.p, D.lvl_node_off.p, D.lvl_edge_off.p, D.dnode.p, D.dedge.p,
+                     D.cur_n.p, D.cur_e.p, D.err.p};
+    launch_dag_layout(c, la, n);
+    int err = 0;
+    HIP_OK(hipMemcpyAsync(&err, D.err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (err) throw std::invalid_argument("malformed DAG (node range, slot counts, source, parent order or slots)");
+    const double t_up = now_s();
+    D.costs.alloc(mx_nodes * ML);
+    D.lo.alloc(mx_nodes);
+    D.emit.alloc(mx_nodes * 9);
+    D.packed.alloc(std::max<int64_t>(1, mx_slots) * ML);
+    D.best.alloc(std::max<int64_t>(1, mx_slots) * ML);
+    D.cost_edges.alloc(std::max<int64_t>(1, mx_edges) * LL);
+    D.best_change.alloc(std::max<int64_t>(1, mx_edges) * LL);
+    NetsDev& N = c->nets;
+    N.dag_routes.push_back(new DevBuf<int>());
+    N.dag_routes.back()->alloc(route_slot[n]);
+    int* rbase = N.dag_routes.back()->p;
+    const double tp = now_s();
+
+    // Batch b: DAG routing (scattering its wires, vias and dirty flags), the
+    // rip-up scatter of batch b+1 from its current routes, then one fused
+    // pass: commit b, rip up b+1, costs.  Stamps follow the plain sequence
+    // (rip-up, then commit, per batch).
+    {
+        BatchedState bs(c);
+        int s1 = ++c->stamp;
+        launch_rip_scatter(c, D.nets.p + batches[0].net0, batches[0].net1 - batches[0].net0);
+        launch_track_fused(c, 0, s1, true);
+        for (size_t b = 0; b < batches.size(); b++) {
+            const int s2 = ++c->stamp;
+            launch_dag_batch_wave(c, batches[b], s2);
+            if (b + 1 < batches.size()) {
+                const DagBatch& Bn = batches[b + 1];
+                s1 = ++c->stamp;
+                launch_rip_scatter(c, D.nets.p + Bn.net0, Bn.net1 - Bn.net0);
+                launch_track_fused(c, s2, s1, true);
+            } else {
+                launch_track_fused(c, s2, 0, false);
+            }
+        }
+        bs.check();
+    }
+    HIP_OK(hipStreamSynchronize(s));
+    for (int64_t r = 0; r < n; r++) {
+        N.h_rptr[rnets[r]] = rbase + route_slot[r];
+        N.h_rcap[rnets[r]] = route_slot[r + 1] - route_slot[r];
+    }
+    const double t2 = now_s();
+    c->last_dag_prep_s = tp - t1;
+    c->last_dag_prep_parts[0] = t_meta - t1;     // routing-order bases, batch bounds, small uploads
+    c->last_dag_prep_parts[1] = t_up - t_meta;   // device layout
+    c->last_dag_prep_parts[2] = t_helper;        // DAG upload and measuring (overlaps the scheduler)
+    c->last_dag_prep_parts[3] = tp - t_up;       // scratch allocation
+    if (st) *st = igr_pass_stats{nb, t1 - t0, t2 - t1, n};
+}
+
+} // namespace
+
 int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, const igr_dag_desc* dg,
                  igr_pass_stats* st) {
     return guard([&]() {
@@ -408,6 +669,10 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
         if (!dg || dg->n != n) throw std::invalid_argument("one DAG per listed net");
         if (n == 0) {
             if (st) *st = igr_pass_stats{0, 0, 0, 0};
+            return;
+        }
+        if (form == IGR_FORM_BATCHED) {
+            dag_pass_batched(c, nets, n, mode, dg, st);
             return;
         }
         const int XY = c->g.X * c->g.Y, L = c->g.L, LL = L * L, ML = L + 1;
@@ -435,8 +700,7 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
         D.pt_off.upload(dg->pt_off, n + 1, s);
         D.pt.upload(dg->pts, dg->pt_off[n], s);
         std::vector<int> batch_of(n);
-        int nb = form == IGR_FORM_PLAIN ? run_schedule(c, nets, n, mode, batch_of.data(), D.pt_off.p, D.pt.p)
-                                        : run_schedule_batched(c, nets, n, mode, batch_of.data(), D.pt_off.p, D.pt.p);
+        int nb = run_schedule(c, nets, n, mode, batch_of.data(), D.pt_off.p, D.pt.p);
         double t1 = now_s();
 
         // Routing order: batch nb-1 first, inside a batch decreasing list
@@ -513,15 +777,10 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
             nbase[r + 1] += nbase[r]; ebase[r + 1] += ebase[r]; sbase[r + 1] += sbase[r];
             lbase[r + 1] += lbase[r]; route_slot[r + 1] += route_slot[r];
         }
-        const int64_t NN = nbase[n], NE = ebase[n], NS = sbase[n], NL = lbase[n];
+        const int64_t NN = nbase[n], NE = ebase[n], NS = sbase[n];
         std::vector<int> rnets(n), node(NN), par(NE), slot(NE), child_num(NN), loc(NN), enode(NE);
         std::vector<int64_t> par_off(NN + 1), csum(NN + 1);
-        const bool batched = form == IGR_FORM_BATCHED;
-        std::vector<int> nlevels(batched ? n : 0), dnode(batched ? NN : 0);
-        std::vector<int64_t> lvl_node_off(batched ? NL + n : 0), lvl_edge_off(batched ? NL + n : 0),
-            dedge(batched ? NE : 0), nl_off(batched ? n : 0);
         pfor(n, [&](int64_t a0, int64_t a1) {
-            std::vector<int64_t> cnt;
             for (int64_t r = a0; r < a1; r++) {
                 const int64_t i = rorder[r], b0 = dg->node_off[i], b1 = dg->node_off[i + 1];
                 const int64_t g0 = nbase[r];
@@ -541,29 +800,6 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
                         enode[e] = (int)gk;
                     }
                 }
-                if (!batched) continue;
-                // Depth levels of this net: nodes by depth (stable), then the
-                // parent edges of each level's nodes.
-                const int nl = (int)(lbase[r + 1] - lbase[r]);
-                const int64_t lo = lbase[r] + r;  // nl+1 boundaries per net
-                nlevels[r] = nl;
-                nl_off[r] = lo;
-                cnt.assign(nl + 1, 0);
-                for (int64_t k = b0; k < b1; k++) cnt[dg->depth[k] + 1]++;
-                for (int d = 0; d < nl; d++) cnt[d + 1] += cnt[d];
-                for (int d = 0; d <= nl; d++) lvl_node_off[lo + d] = g0 + cnt[d];
-                std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
-                for (int64_t k = b0; k < b1; k++) dnode[g0 + pos[dg->depth[k]]++] = (int)(g0 + (k - b0));
-                int64_t ed = ebase[r];
-                for (int d = 0; d < nl; d++) {
-                    lvl_edge_off[lo + d] = ed;
-                    for (int64_t q = lvl_node_off[lo + d]; q < lvl_node_off[lo + d + 1]; q++) {
-                        const int64_t gk = dnode[q], k = b0 + (gk - g0);
-                        const int64_t eb = par_off[gk];
-                        for (int64_t t = 0; t < dg->par_off[k + 1] - dg->par_off[k]; t++) dedge[ed++] = eb + t;
-                    }
-                }
-                lvl_edge_off[lo + nl] = ed;
             }
         });
         par_off[NN] = NE;
@@ -591,7 +827,7 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
             }
             B.levels = maxd + 1;
             B.lvl0 = (int64_t)lvl_off.size() - 1;
-            if (!batched) {
+            {
                 // All nodes of the batch bucketed by depth (stable in node order).
                 std::vector<int64_t> cnt(B.levels + 1, 0);
                 for (int64_t r = B.net0; r < B.net1; r++) {
@@ -626,16 +862,7 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
         D.par_off.upload(par_off.data(), par_off.size(), s);
         D.csum.upload(csum.data(), csum.size(), s);
         D.route_slot.upload(route_slot.data(), route_slot.size(), s);
-        if (batched) {
-            D.nl_off.upload(nl_off.data(), nl_off.size(), s);
-            D.nlevels.upload(nlevels.data(), nlevels.size(), s);
-            D.lvl_node_off.upload(lvl_node_off.data(), lvl_node_off.size(), s);
-            D.lvl_edge_off.upload(lvl_edge_off.data(), lvl_edge_off.size(), s);
-            D.dnode.upload(dnode.data(), dnode.size(), s);
-            D.dedge.upload(dedge.data(), dedge.size(), s);
-        } else {
-            D.lvl_nodes.upload(lvl_nodes.data(), lvl_nodes.size(), s);
-        }
+        D.lvl_nodes.upload(lvl_nodes.data(), lvl_nodes.size(), s);
         HIP_OK(hipStreamSynchronize(s));
         const double t_up = now_s();
         D.costs.alloc(mx_nodes * ML);
@@ -651,44 +878,20 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
         int* rbase = N.dag_routes.back()->p;
         double tp = now_s();
 
-        if (!batched) {
-            for (const DagBatch& B : batches) {
-                const int* bn = D.nets.p + B.net0;
-                const int64_t cnt = B.net1 - B.net0;
-                const int s1 = ++c->stamp;
-                launch_rip_wire(c, bn, cnt, s1);
-                launch_via_commit(c, bn, cnt, s1, -1);
-                const int s2 = ++c->stamp;
-                launch_update_cost(c);
-                launch_presum(c);
-                launch_dag_batch_plain(c, B, s2, lvl_off);
-                launch_dag_compact(c, B);
-                launch_wire_commit(c, s2);
-                launch_via_commit(c, bn, cnt, s2, 1);
-                HIP_OK(hipStreamSynchronize(s));
-            }
-        } else {
-            // Batch b: DAG routing (scattering its wires, vias and dirty
-            // flags), the rip-up scatter of batch b+1 from its current routes,
-            // then one fused pass: commit b, rip up b+1, costs.  Stamps follow
-            // the plain sequence (rip-up, then commit, per batch).
-            BatchedState bs(c);
-            int s1 = ++c->stamp;
-            launch_rip_scatter(c, D.nets.p + batches[0].net0, batches[0].net1 - batches[0].net0);
-            launch_track_fused(c, 0, s1, true);
-            for (size_t b = 0; b < batches.size(); b++) {
-                const int s2 = ++c->stamp;
-                launch_dag_batch_wave(c, batches[b], s2);
-                if (b + 1 < batches.size()) {
-                    const DagBatch& Bn = batches[b + 1];
-                    s1 = ++c->stamp;
-                    launch_rip_scatter(c, D.nets.p + Bn.net0, Bn.net1 - Bn.net0);
-                    launch_track_fused(c, s2, s1, true);
-                } else {
-                    launch_track_fused(c, s2, 0, false);
-                }
-            }
-            bs.check();
+        for (const DagBatch& B : batches) {
+            const int* bn = D.nets.p + B.net0;
+            const int64_t cnt = B.net1 - B.net0;
+            const int s1 = ++c->stamp;
+            launch_rip_wire(c, bn, cnt, s1);
+            launch_via_commit(c, bn, cnt, s1, -1);
+            const int s2 = ++c->stamp;
+            launch_update_cost(c);
+            launch_presum(c);
+            launch_dag_batch_plain(c, B, s2, lvl_off);
+            launch_dag_compact(c, B);
+            launch_wire_commit(c, s2);
+            launch_via_commit(c, bn, cnt, s2, 1);
+            HIP_OK(hipStreamSynchronize(s));
         }
         HIP_OK(hipStreamSynchronize(s));
         for (int64_t r = 0; r < n; r++) {

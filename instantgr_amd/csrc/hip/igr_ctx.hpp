@@ -93,6 +93,44 @@ struct DagDev {
     // Batched form: depth levels of every net (nodes and their parent edges).
     DevBuf<int64_t> nl_off, lvl_node_off, lvl_edge_off, dedge;
     DevBuf<int> nlevels, dnode;
+    // Batched form: the DAGs as uploaded (net list order, local parents), the
+    // per-routed-net bases and the level cursors of the device layout.
+    DevBuf<int64_t> r_node_off, r_par_off, ebase, sbase, lbase, cur_n, cur_e;
+    DevBuf<int> r_nodes, r_par, r_slot, r_child_num, r_depth, rorder, err;
+};
+
+// Arguments of the device DAG layout (one workgroup per routed net).
+struct DagLayoutArgs {
+    const int64_t* node_off;      // uploaded DAGs, net list order
+    const int* nodes;
+    const int64_t* par_off;
+    const int* par;
+    const int* slot;
+    const int* child_num;
+    const int* depth;
+    const int* rorder;            // routed net -> list position
+    const int64_t* nbase;         // per routed net: first node, edge, slot, level
+    const int64_t* ebase;
+    const int64_t* sbase;
+    const int64_t* lbase;
+    int node_limit;               // (L + 1) * X * Y
+    int* node;
+    int* o_child_num;
+    int* loc;
+    int64_t* csum;
+    int64_t* o_par_off;
+    int* o_par;
+    int* o_slot;
+    int* enode;
+    int64_t* nl_off;
+    int* nlevels;
+    int64_t* lvl_node_off;
+    int64_t* lvl_edge_off;
+    int* dnode;
+    int64_t* dedge;
+    int64_t* cur_n;
+    int64_t* cur_e;
+    int* err;
 };
 
 struct Scratch {
@@ -144,6 +182,7 @@ struct igr_ctx {
     igr_dev::KProf prof;
     igr_dev::GridDev g{};
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;   // uploads that overlap the compute stream
     int device = 0;
     igr_dev::DevBuf<double> w;
     igr_dev::DevBuf<int> xlen, ylen;
@@ -221,6 +260,8 @@ void launch_dag_batch_plain(igr_ctx* c, const DagBatch& b, int stamp, const std:
 void launch_dag_compact(igr_ctx* c, const DagBatch& b);
 // Batched form: one workgroup per net, all depth levels in one launch.
 void launch_dag_batch_wave(igr_ctx* c, const DagBatch& b, int stamp);
+// Batched form: the device layout of a pass's DAGs.
+void launch_dag_layout(igr_ctx* c, const DagLayoutArgs& a, int64_t n);
 void launch_expf(const float* in, float* out, int64_t n, hipStream_t s);
 // Device-to-device copy of 'bytes' repeated 'reps' times; returns GB/s
 // (bytes read + written).
