@@ -20,8 +20,9 @@
 // reference's MAX_LAYER-1 with its 9 routing layers.
 //
 // Each worker thread reuses one builder (no per-net allocations once warm)
-// and writes a chunk of nets into its own buffer; the chunks are then copied
-// into the final arrays in parallel.
+// and writes a chunk of nets into a chunk buffer; the chunks are then copied
+// into the final arrays in parallel.  Chunk buffers and final arrays keep
+// their memory across builds, so later passes neither zero nor fault it in.
 #include "dag.hpp"
 
 #include <algorithm>
@@ -61,7 +62,26 @@ struct Builder {
     std::vector<std::vector<int>> mir_buf[2], place_buf[2];
     // scratch
     std::vector<std::pair<float, int>> road;
-    std::vector<int> q, visited, pcnt, poff, plist, visit, seq, st;
+    std::vector<int> q, visited, pcnt, poff, plist, visit, seq, st, hset;
+
+    // The scheduler points of the net without duplicates, in order of first
+    // occurrence (the scheduler does not depend on their order): an
+    // open-addressing set sized to twice the list.
+    void dedup_points() {
+        size_t cap = 16;
+        while (cap < 2 * points.size()) cap <<= 1;
+        hset.assign(cap, -1);
+        size_t w = 0;
+        for (size_t k = 0; k < points.size(); k++) {
+            const int p = points[k];
+            size_t h = ((uint32_t)p * 2654435761u) & (cap - 1);
+            while (hset[h] != -1 && hset[h] != p) h = (h + 1) & (cap - 1);
+            if (hset[h] == p) continue;
+            hset[h] = p;
+            points[w++] = p;
+        }
+        points.resize(w);
+    }
 
     Builder(const Grid& g_, const float* xs, const float* ys, int mode_)
         : g(g_), xsum(xs), ysum(ys), mode(mode_), X(g_.X), Y(g_.Y), XYc(g_.X * g_.Y), L(g_.L) {
@@ -426,8 +446,7 @@ struct Builder {
         edges.clear();
         points.clear();
         dfs(root, -1, -1, -1, 0, 0);
-        std::sort(points.begin(), points.end());
-        points.erase(std::unique(points.begin(), points.end()), points.end());
+        dedup_points();
 
         // Parent lists in creation order (CSR by child).
         const int n = (int)nodes.size();
@@ -524,7 +543,8 @@ void build_dags(const Grid& g, const Subnets& s, const int* nets, int64_t n, con
     if (threads < 1) threads = 1;
     const int64_t CH = 512;
     const int64_t nch = (n + CH - 1) / CH;
-    std::vector<DagChunk> chunks(nch);
+    std::vector<DagChunk>& chunks = out.work;
+    if ((int64_t)chunks.size() < nch) chunks.resize(nch);
     std::atomic<int64_t> next(0);
     std::vector<std::string> errs(threads);
     std::vector<std::thread> pool;
@@ -536,6 +556,9 @@ void build_dags(const Grid& g, const Subnets& s, const int* nets, int64_t n, con
                     int64_t c = next.fetch_add(1);
                     if (c >= nch) break;
                     DagChunk& ck = chunks[c];
+                    for (auto* v : {&ck.node, &ck.child_num, &ck.depth, &ck.npar, &ck.par, &ck.slot, &ck.points,
+                                    &ck.nodes_per_net, &ck.points_per_net})
+                        v->clear();
                     for (auto* v : {&ck.node, &ck.child_num, &ck.depth, &ck.npar}) v->reserve(CH * 64);
                     for (auto* v : {&ck.par, &ck.slot, &ck.points}) v->reserve(CH * 96);
                     for (int64_t i = c * CH; i < std::min(n, (c + 1) * CH); i++) {
@@ -559,10 +582,9 @@ void build_dags(const Grid& g, const Subnets& s, const int* nets, int64_t n, con
         ce[c + 1] = ce[c] + (int64_t)chunks[c].par.size();
         cp[c + 1] = cp[c] + (int64_t)chunks[c].points.size();
     }
-    out = DagSet();
     out.n = n;
-    out.node_off.assign(n + 1, 0);
-    out.pt_off.assign(n + 1, 0);
+    out.node_off.resize(n + 1);
+    out.pt_off.resize(n + 1);
     out.node.resize(cn[nch]);
     out.child_num.resize(cn[nch]);
     out.depth.resize(cn[nch]);

@@ -2,6 +2,10 @@
 #pragma once
 #include "design.hpp"
 
+#include <memory>
+#include <utility>
+#include <vector>
+
 namespace igr {
 
 // One net's DAG in topological order (parents before children): node i has
@@ -13,18 +17,43 @@ struct DagNet {
     std::vector<int> node, par_off, par, slot, child_num, depth, points;
 };
 
-// DAGs of several nets concatenated; par_off is global over all nodes and
-// par holds local (per-net) node indices.
-struct DagSet {
-    int64_t n = 0;
-    std::vector<int64_t> node_off, par_off, pt_off;
-    std::vector<int> node, par, slot, child_num, depth, pt;
+// Allocator that leaves elements default-initialised on resize (no zero
+// fill): the DAG arrays are overwritten in full right after they are sized.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <class U>
+    NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+    template <class U>
+    void construct(U* p) noexcept {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template <class U, class... A>
+    void construct(U* p, A&&... a) {
+        ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+    }
 };
+template <class T>
+using RawVec = std::vector<T, NoInitAlloc<T>>;
 
 // Consecutive nets' DAGs as one builder thread produces them (par local,
 // npar = parent count per node).
 struct DagChunk {
     std::vector<int> node, child_num, depth, npar, par, slot, points, nodes_per_net, points_per_net;
+};
+
+// DAGs of several nets concatenated; par_off is global over all nodes and
+// par holds local (per-net) node indices.  The arrays and the per-chunk work
+// buffers keep their capacity from one build to the next.
+struct DagSet {
+    int64_t n = 0;
+    RawVec<int64_t> node_off, par_off, pt_off;
+    RawVec<int> node, par, slot, child_num, depth, pt;
+    std::vector<DagChunk> work;
 };
 
 void build_dag(const Grid& g, const float* xsum, const float* ysum, int mode, const int* tnode, int tn,
