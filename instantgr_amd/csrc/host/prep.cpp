@@ -1,5 +1,6 @@
 // Host preparation of the routing passes and the route file writer.
 #include "prep.hpp"
+#include "par.hpp"
 
 #include <algorithm>
 #include <cstdio>
@@ -59,7 +60,9 @@ void prepare_pattern_inputs(const Grid& g, const Subnets& s, PatternInputs& pi, 
             }
         }
     };
-    for (int64_t i = 0; i < S; i++) foot(i, nullptr, nullptr, nullptr, npt[i], nh[i], nv[i]);
+    parallel_ranges(S, threads, [&](int64_t a, int64_t b, int) {
+        for (int64_t i = a; i < b; i++) foot(i, nullptr, nullptr, nullptr, npt[i], nh[i], nv[i]);
+    });
     pi.pt_off.assign(S + 1, 0);
     pi.hs_off.assign(S + 1, 0);
     pi.vs_off.assign(S + 1, 0);
@@ -73,12 +76,10 @@ void prepare_pattern_inputs(const Grid& g, const Subnets& s, PatternInputs& pi, 
     pi.vs.resize(2 * pi.vs_off[S]);
 
     if (threads < 1) threads = 1;
-    std::vector<std::thread> pool;
     std::vector<int> maxn(threads, 0);
-    for (int t = 0; t < threads; t++)
-        pool.emplace_back([&, t]() {
+    parallel_ranges(S, threads, [&](int64_t a, int64_t b_, int t) {
             std::vector<std::tuple<int, int, int>> st;   // (node, parent node, parent idx)
-            for (int64_t i = t; i < S; i += threads) {
+            for (int64_t i = a; i < b_; i++) {
                 int64_t cp, ch, cv;
                 foot(i, &pi.pt[pi.pt_off[i]], &pi.hs[2 * pi.hs_off[i]], &pi.vs[2 * pi.vs_off[i]], cp, ch, cv);
                 const int64_t b = s.node_off[i], n = s.node_off[i + 1] - b, o = pi.node_off[i];
@@ -100,11 +101,16 @@ void prepare_pattern_inputs(const Grid& g, const Subnets& s, PatternInputs& pi, 
                 if (cnt != n) throw std::runtime_error("subnet tree is not connected");
             }
         });
-    for (auto& th : pool) th.join();
     pi.max_nodes = *std::max_element(maxn.begin(), maxn.end());
+    // Routing order: decreasing hpwl, ties in index order (a stable counting
+    // sort; the same order as a stable sort on hpwl).
+    int maxh = 0;
+    for (int64_t i = 0; i < S; i++) maxh = std::max(maxh, s.hpwl[i]);
+    std::vector<int64_t> start(maxh + 2, 0);
+    for (int64_t i = 0; i < S; i++) start[maxh - s.hpwl[i] + 1]++;
+    for (int h = 0; h <= maxh; h++) start[h + 1] += start[h];
     pi.order.resize(S);
-    std::iota(pi.order.begin(), pi.order.end(), 0);
-    std::stable_sort(pi.order.begin(), pi.order.end(), [&](int a, int b) { return s.hpwl[a] > s.hpwl[b]; });
+    for (int64_t i = 0; i < S; i++) pi.order[start[maxh - s.hpwl[i]]++] = (int)i;
 }
 
 void write_routes(const std::string& path, const Design& d, const NetPins& np, const Grid& g,

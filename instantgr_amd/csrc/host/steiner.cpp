@@ -15,8 +15,10 @@
 // Nets with more than 12 pins are split into subnets with the union-find pass
 // of src/database_cuda.hpp:2111-2228.
 #include "design.hpp"
+#include "par.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <functional>
 #include <numeric>
 #include <stdexcept>
@@ -262,11 +264,14 @@ void build_subnets(const Design& d, const NetPins& np, const Grid& g, Subnets& s
     for (int i = 0; i + 1 < g.Y; i++) C.dry[i + 1] = C.dry[i] + g.ylen[i];
     std::vector<std::vector<SubOut>> per(N);
     if (threads < 1) threads = 1;
-    std::vector<std::thread> pool;
-    for (int t = 0; t < threads; t++)
-        pool.emplace_back([&, t]() {
-            std::vector<int> pins;
-            for (int64_t n = t; n < N; n += threads) {
+    std::atomic<int64_t> next(0);
+    const int64_t CH = 256;   // nets per work item (heavy nets are spread by the dynamic pull)
+    parallel_ranges(threads, threads, [&](int64_t, int64_t, int) {
+        std::vector<int> pins;
+        for (;;) {
+            const int64_t c0 = next.fetch_add(CH);
+            if (c0 >= N) break;
+            for (int64_t n = c0; n < std::min(N, c0 + CH); n++) {
                 int64_t b = np.pin_off[n], e = np.pin_off[n + 1];
                 if (e - b <= 1) continue;
                 pins.assign(np.pins.begin() + b, np.pins.begin() + e);
@@ -276,37 +281,76 @@ void build_subnets(const Design& d, const NetPins& np, const Grid& g, Subnets& s
                     if (p >= XY) p -= XY;
                 split_net(C, g, pins, per[n]);
             }
-        });
-    for (auto& th : pool) th.join();
-
-    s = Subnets();
-    s.net_sub_off.assign(N + 1, 0);
-    s.pin_off.assign(1, 0);
-    s.node_off.assign(1, 0);
-    s.adj_off.assign(1, 0);
-    for (int64_t n = 0; n < N; n++) {
-        for (auto& so : per[n]) {
-            s.net_sub.push_back((int)s.count++);
-            s.orig.push_back((int)n);
-            int minx = g.X, maxx = 0, miny = g.Y, maxy = 0;
-            for (int p : so.pins) {
-                int x = p / g.Y % g.X, y = p % g.Y;
-                minx = std::min(minx, x); maxx = std::max(maxx, x);
-                miny = std::min(miny, y); maxy = std::max(maxy, y);
-            }
-            s.hpwl.push_back(maxx - minx + maxy - miny);
-            s.pins.insert(s.pins.end(), so.pins.begin(), so.pins.end());
-            s.pin_off.push_back((int64_t)s.pins.size());
-            for (size_t i = 0; i < so.node.size(); i++) {
-                s.node.push_back(so.node[i]);
-                s.adj.insert(s.adj.end(), so.adj[i].begin(), so.adj[i].end());
-                s.adj_off.push_back((int64_t)s.adj.size());
-            }
-            s.node_off.push_back((int64_t)s.node.size());
         }
-        s.net_sub_off[n + 1] = (int64_t)s.net_sub.size();
-        std::vector<SubOut>().swap(per[n]);
+    });
+
+    // Sizes per net, offsets, then a parallel fill in net order.
+    std::vector<int64_t> c_sub(N + 1, 0), c_pin(N + 1, 0), c_node(N + 1, 0), c_adj(N + 1, 0);
+    parallel_ranges(N, threads, [&](int64_t a, int64_t b, int) {
+        for (int64_t n = a; n < b; n++) {
+            int64_t ps = 0, ns = 0, as = 0;
+            for (const auto& so : per[n]) {
+                ps += (int64_t)so.pins.size();
+                ns += (int64_t)so.node.size();
+                for (const auto& v : so.adj) as += (int64_t)v.size();
+            }
+            c_sub[n + 1] = (int64_t)per[n].size();
+            c_pin[n + 1] = ps;
+            c_node[n + 1] = ns;
+            c_adj[n + 1] = as;
+        }
+    });
+    for (int64_t n = 0; n < N; n++) {
+        c_sub[n + 1] += c_sub[n];
+        c_pin[n + 1] += c_pin[n];
+        c_node[n + 1] += c_node[n];
+        c_adj[n + 1] += c_adj[n];
     }
+    const int64_t S = c_sub[N], NN = c_node[N];
+    s = Subnets();
+    s.count = S;
+    s.net_sub_off = c_sub;
+    s.net_sub.resize(S);
+    s.orig.resize(S);
+    s.hpwl.resize(S);
+    s.pin_off.resize(S + 1);
+    s.pins.resize(c_pin[N]);
+    s.node_off.resize(S + 1);
+    s.node.resize(NN);
+    s.adj_off.resize(NN + 1);
+    s.adj.resize(c_adj[N]);
+    parallel_ranges(N, threads, [&](int64_t a, int64_t b, int) {
+        for (int64_t n = a; n < b; n++) {
+            int64_t k = c_sub[n], pc = c_pin[n], nc = c_node[n], ac = c_adj[n];
+            for (const auto& so : per[n]) {
+                s.net_sub[k] = (int)k;
+                s.orig[k] = (int)n;
+                int minx = g.X, maxx = 0, miny = g.Y, maxy = 0;
+                for (int p : so.pins) {
+                    int x = p / g.Y % g.X, y = p % g.Y;
+                    minx = std::min(minx, x); maxx = std::max(maxx, x);
+                    miny = std::min(miny, y); maxy = std::max(maxy, y);
+                }
+                s.hpwl[k] = maxx - minx + maxy - miny;
+                s.pin_off[k] = pc;
+                std::copy(so.pins.begin(), so.pins.end(), s.pins.begin() + pc);
+                pc += (int64_t)so.pins.size();
+                s.node_off[k] = nc;
+                for (size_t i = 0; i < so.node.size(); i++) {
+                    s.node[nc] = so.node[i];
+                    s.adj_off[nc] = ac;
+                    std::copy(so.adj[i].begin(), so.adj[i].end(), s.adj.begin() + ac);
+                    ac += (int64_t)so.adj[i].size();
+                    nc++;
+                }
+                k++;
+            }
+            std::vector<SubOut>().swap(per[n]);
+        }
+    });
+    s.pin_off[S] = c_pin[N];
+    s.node_off[S] = NN;
+    s.adj_off[NN] = c_adj[N];
 }
 
 } // namespace igr
