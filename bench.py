@@ -52,6 +52,10 @@ def parse():
     return ap.parse_args()
 
 
+def _progress(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
 def one_step(design, form, rrr, threads, profile=False):
     from instantgr_amd import gpu
     t0 = time.perf_counter()
@@ -231,20 +235,24 @@ def main():
     from instantgr_amd import gpu
     from instantgr_amd.host import Design
     copy_gbs = gpu.copy_probe(1 << 30, 20) if a.copy_probe else None
+    t_gen = time.perf_counter()
     design = Design.preset(a.config, nets=a.nets)
+    _progress("design generated in %.1f s" % (time.perf_counter() - t_gen))
     forms = [f.strip() for f in a.forms.split(",") if f.strip()]
     form_id = {"plain": gpu.FORM_PLAIN, "batched": gpu.FORM_BATCHED}
     results = {}
     routes = {}
     for f in forms:
-        for _ in range(a.warmup):
+        for w in range(a.warmup):
             r, _st = one_step(design, form_id[f], a.rrr, a.threads)
             r.close()
+            _progress("%s warmup %d: %.3f s" % (f, w, _st["step_s"]))
         times = []
         last = None
         for k in range(a.steps):
             r, st = one_step(design, form_id[f], a.rrr, a.threads, profile=(k == a.steps - 1))
             times.append(st["step_s"])
+            _progress("%s step %d: %.3f s" % (f, k, st["step_s"]))
             if last is not None:
                 last[0].close()
             last = (r, st)

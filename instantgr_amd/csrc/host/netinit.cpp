@@ -11,10 +11,9 @@
 
 #include <algorithm>
 #include <cstdlib>
-#include <map>
-#include <set>
 #include <thread>
-#include <unordered_map>
+#include <utility>
+#include <vector>
 
 namespace igr {
 namespace {
@@ -24,6 +23,18 @@ struct PinSel {
     std::vector<int> extra;
 };
 
+// Per-thread tables of the greedy clustering, reused across nets.  The maps
+// over the grid (position -> index, chosen positions per x and per y) are
+// cleared after each net at the entries it touched.
+struct Scratch {
+    std::vector<int> pidmap;           // 2-D position -> index among the net's positions, or -1
+    std::vector<int> xc, yc;           // chosen positions per x / per y
+    std::vector<int> pos;              // index -> 2-D position
+    std::vector<int> apid;             // access point -> position index
+    std::vector<int> pdist, cnt;       // per position: 2x distance to the centre; round count
+    std::vector<char> in2d;            // position already chosen
+};
+
 void init_net(const Design& d, int64_t n, PinSel& out) {
     const int X = d.X, Y = d.Y, XY = d.X * d.Y;
     const int64_t p0 = d.pin_off[n], num = d.pin_off[n + 1] - p0;
@@ -31,7 +42,9 @@ void init_net(const Design& d, int64_t n, PinSel& out) {
     const int c2x = bb[0] + bb[1], c2y = bb[2] + bb[3];
     auto apc = [&](int64_t i) { return (int)(d.ap_off[p0 + i + 1] - d.ap_off[p0 + i]); };
     auto apv = [&](int64_t i, int j) { return d.ap[d.ap_off[p0 + i] + j]; };  // j is 0-based
-    std::set<int> p, p2D;
+    // Chosen access points and their 2-D positions (sets kept as vectors,
+    // sorted and de-duplicated at the end).
+    std::vector<int> p, p2D;
 
     if (num <= 4) {
         std::vector<int> idx(num, 0), best(num, 0);
@@ -67,82 +80,127 @@ void init_net(const Design& d, int64_t n, PinSel& out) {
         }
         for (int64_t i = 0; i < num; i++) {
             int v = apv(i, best[i]);
-            p.insert(v);
-            p2D.insert(v % XY);
+            p.push_back(v);
+            p2D.push_back(v % XY);
         }
     } else {
-        std::map<int, int> x_counts, y_counts;
+        // Greedy overlap clustering (src/database.hpp:92-197) on local tables:
+        // each access point gets the index of its 2-D position and of its x
+        // and y coordinate among the net's distinct values, so a round's
+        // counting is array work instead of hash and tree lookups.
+        thread_local Scratch S;
+        if (S.pidmap.size() != (size_t)XY) {
+            S.pidmap.assign(XY, -1);
+            S.xc.assign(X, 0);
+            S.yc.assign(Y, 0);
+        }
+        const int64_t a0 = d.ap_off[p0], A = d.ap_off[p0 + num] - a0;
+        S.pos.clear();
+        S.apid.resize(A);
+        for (int64_t a = 0; a < A; a++) {
+            const int v = d.ap[a0 + a] % XY;
+            int& id = S.pidmap[v];
+            if (id < 0) {
+                id = (int)S.pos.size();
+                S.pos.push_back(v);
+            }
+            S.apid[a] = id;
+        }
+        const int P = (int)S.pos.size();
+        S.pdist.resize(P);
+        for (int q = 0; q < P; q++) {
+            const int v = S.pos[q];
+            S.pdist[q] = std::abs(v / Y * 2 - c2x) + std::abs(v % Y * 2 - c2y);
+        }
+        S.in2d.assign(P, 0);
+        S.cnt.resize(P);
+        auto cr_of = [&](int q) { return S.xc[S.pos[q] / Y] + S.yc[S.pos[q] % Y]; };
+        auto choose = [&](int q) {
+            S.in2d[q] = 1;
+            p2D.push_back(S.pos[q]);
+            S.xc[S.pos[q] / Y]++;
+            S.yc[S.pos[q] % Y]++;
+        };
         std::vector<char> visited(num, 0);
+        auto ap_first = [&](int64_t i) { return d.ap_off[p0 + i] - a0; };
         for (int64_t round = 0; round < num; round++) {
-            int max_pos = -1;
-            size_t cnt_repeat = 0;
-            int max_coord_repeat = -1, min_dist = 1000000000;
-            std::unordered_map<int, std::vector<int>> cnt;
+            int max_q = -1;
+            int cnt_repeat = 0, max_coord_repeat = -1, min_dist = 1000000000;
+            std::fill(S.cnt.begin(), S.cnt.end(), 0);
             for (int64_t i = 0; i < num; i++) {
                 if (visited[i]) continue;
-                for (int j = 0; j < apc(i); j++) {
-                    int pin2d = apv(i, j) % XY;
-                    auto& lst = cnt[pin2d];
-                    lst.push_back((int)i);
-                    int dist = std::abs(pin2d / Y * 2 - c2x) + std::abs(pin2d % Y * 2 - c2y);
-                    int cr = x_counts[pin2d / Y] + y_counts[pin2d % Y];
-                    if (lst.size() > cnt_repeat) {
-                        max_pos = pin2d; cnt_repeat = lst.size(); max_coord_repeat = cr; min_dist = dist;
-                    } else if (lst.size() == cnt_repeat && cr > max_coord_repeat) {
-                        max_pos = pin2d; max_coord_repeat = cr; min_dist = dist;
-                    } else if (lst.size() == cnt_repeat && cr == max_coord_repeat && dist < min_dist) {
-                        max_pos = pin2d; min_dist = dist;
+                for (int64_t a = ap_first(i); a < ap_first(i + 1); a++) {
+                    const int q = S.apid[a];
+                    const int c = ++S.cnt[q];
+                    const int dist = S.pdist[q];
+                    const int cr = cr_of(q);
+                    if (c > cnt_repeat) {
+                        max_q = q; cnt_repeat = c; max_coord_repeat = cr; min_dist = dist;
+                    } else if (c == cnt_repeat && cr > max_coord_repeat) {
+                        max_q = q; max_coord_repeat = cr; min_dist = dist;
+                    } else if (c == cnt_repeat && cr == max_coord_repeat && dist < min_dist) {
+                        max_q = q; min_dist = dist;
                     }
                 }
             }
             if (cnt_repeat <= 1) break;
-            if (!p2D.count(max_pos)) {
-                p2D.insert(max_pos);
-                x_counts[max_pos / Y % X]++;
-                y_counts[max_pos % Y]++;
-            }
-            for (int pin_id : cnt[max_pos]) {
-                if (visited[pin_id]) continue;
-                visited[pin_id] = 1;
-                for (int j = 0; j < apc(pin_id); j++)
-                    if (apv(pin_id, j) % XY == max_pos) { p.insert(apv(pin_id, j)); break; }
+            if (!S.in2d[max_q]) choose(max_q);
+            // Every unvisited pin with an access point at the chosen position
+            // takes its first such access point.
+            for (int64_t i = 0; i < num; i++) {
+                if (visited[i]) continue;
+                for (int64_t a = ap_first(i); a < ap_first(i + 1); a++)
+                    if (S.apid[a] == max_q) {
+                        visited[i] = 1;
+                        p.push_back(d.ap[a0 + a]);
+                        break;
+                    }
             }
         }
         for (int64_t i = 0; i < num; i++) {
             if (visited[i]) continue;
-            int min_dist = 1000000000, selected = -1, max_overlap = 0;
-            for (int j = 0; j < apc(i); j++) {
-                int v = apv(i, j);
-                int dist = std::abs(v / Y % X * 2 - c2x) + std::abs(v % Y * 2 - c2y);
-                int pin2d = v % XY;
-                int cr = x_counts[pin2d / Y] + y_counts[pin2d % Y];
-                if (cr > max_overlap) { max_overlap = cr; min_dist = dist; selected = v; }
-                else if (cr == max_overlap && dist < min_dist) { min_dist = dist; selected = v; }
+            int min_dist = 1000000000, selected = -1, sel_q = -1, max_overlap = 0;
+            for (int64_t a = ap_first(i); a < ap_first(i + 1); a++) {
+                const int v = d.ap[a0 + a], q = S.apid[a];
+                const int dist = std::abs(v / Y % X * 2 - c2x) + std::abs(v % Y * 2 - c2y);
+                const int cr = cr_of(q);
+                if (cr > max_overlap) { max_overlap = cr; min_dist = dist; selected = v; sel_q = q; }
+                else if (cr == max_overlap && dist < min_dist) { min_dist = dist; selected = v; sel_q = q; }
             }
-            p.insert(selected);
-            if (!p2D.count(selected % XY)) {
-                p2D.insert(selected % XY);
-                x_counts[selected / Y % X]++;
-                y_counts[selected % Y]++;
+            p.push_back(selected);
+            if (!S.in2d[sel_q]) choose(sel_q);
+        }
+        for (int q = 0; q < P; q++) {
+            S.pidmap[S.pos[q]] = -1;
+            if (S.in2d[q]) {
+                S.xc[S.pos[q] / Y] = 0;
+                S.yc[S.pos[q] % Y] = 0;
             }
         }
     }
+    std::sort(p.begin(), p.end());
+    p.erase(std::unique(p.begin(), p.end()), p.end());
+    std::sort(p2D.begin(), p2D.end());
+    p2D.erase(std::unique(p2D.begin(), p2D.end()), p2D.end());
     out.pins.clear();
     out.extra.clear();
     if (p2D.size() < p.size()) {
-        std::map<int, int> pmax, pmin;
-        for (int e : p) {
-            int pos = e % XY;
-            auto it = pmax.find(pos);
-            if (it == pmax.end()) { pmax[pos] = e; pmin[pos] = e; }
-            else { it->second = std::max(it->second, e); pmin[pos] = std::min(pmin[pos], e); }
-        }
-        for (int e : p2D) {
-            out.pins.push_back(pmax[e]);
-            if (pmin[e] != pmax[e]) { out.extra.push_back(pmin[e]); out.extra.push_back(pmax[e]); }
+        // Several layers at one position: the pin takes the highest, the
+        // lowest and highest become an extra via stack.  Every chosen access
+        // point's position is in p2D, so the groups below follow p2D's order.
+        std::vector<std::pair<int, int>> byPos(p.size());
+        for (size_t k = 0; k < p.size(); k++) byPos[k] = {p[k] % XY, p[k]};
+        std::sort(byPos.begin(), byPos.end());
+        for (size_t k = 0; k < byPos.size();) {
+            size_t e = k;
+            while (e < byPos.size() && byPos[e].first == byPos[k].first) e++;
+            const int lo = byPos[k].second, hi = byPos[e - 1].second;
+            out.pins.push_back(hi);
+            if (lo != hi) { out.extra.push_back(lo); out.extra.push_back(hi); }
+            k = e;
         }
     } else {
-        out.pins.assign(p.begin(), p.end());
+        out.pins = p;
     }
 }
 

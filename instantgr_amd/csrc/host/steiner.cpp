@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <climits>
 #include <functional>
 #include <numeric>
 #include <stdexcept>
@@ -83,25 +84,43 @@ void steiner_tree(const Coords& C, const std::vector<int>& pts, Tree2D& t) {
     for (int i = 0; i < n; i++) { nx[i] = pts[i] / C.Y; ny[i] = pts[i] % C.Y; }
     std::vector<std::pair<int, int>> edges;
     if (n >= 2) {
-        // Prim, ties to the smallest index.
-        std::vector<long long> best(n, -1);
-        std::vector<int> from(n, 0);
-        std::vector<char> in(n, 0);
-        in[0] = 1;
-        for (int j = 1; j < n; j++) best[j] = C.dist(nx[0], ny[0], nx[j], ny[j]);
+        // Prim, ties to the smallest index.  The nodes outside the tree sit
+        // in structure-of-arrays form; one pass per step relaxes them against
+        // the node just added and finds the next by the key (distance << sh)
+        // | index, whose minimum is the nearest node with the smallest index.
+        // The added node leaves by a swap with the last entry.
+        int sh = 1;
+        while ((1 << sh) <= n) sh++;
+        const long long span = C.drx.back() + C.dry.back();
+        if (span >= (1LL << (62 - sh))) throw std::runtime_error("Steiner tree: coordinate span too large");
+        std::vector<long long> rx(n - 1), ry(n - 1), rk(n - 1);
+        std::vector<int> rid(n - 1), rfrom(n - 1, 0);
+        const long long x0 = C.drx[nx[0]], y0 = C.dry[ny[0]];
+        int m = n - 1, pj = 0;
+        for (int r = 0; r < m; r++) {
+            const int k = r + 1;
+            rx[r] = C.drx[nx[k]];
+            ry[r] = C.dry[ny[k]];
+            rid[r] = k;
+            rk[r] = ((std::llabs(x0 - rx[r]) + std::llabs(y0 - ry[r])) << sh) | k;
+            if (rk[r] < rk[pj]) pj = r;
+        }
         std::vector<std::vector<int>> adj(n);
-        for (int it = 1; it < n; it++) {
-            int j = -1;
-            for (int k = 0; k < n; k++)
-                if (!in[k] && (j < 0 || best[k] < best[j])) j = k;
-            in[j] = 1;
-            adj[from[j]].push_back(j);
-            adj[j].push_back(from[j]);
-            for (int k = 0; k < n; k++)
-                if (!in[k]) {
-                    long long dk = C.dist(nx[j], ny[j], nx[k], ny[k]);
-                    if (dk < best[k]) { best[k] = dk; from[k] = j; }
-                }
+        while (m > 0) {
+            const int j = rid[pj], fj = rfrom[pj];
+            const long long jx = rx[pj], jy = ry[pj];
+            adj[fj].push_back(j);
+            adj[j].push_back(fj);
+            m--;
+            rx[pj] = rx[m]; ry[pj] = ry[m]; rk[pj] = rk[m]; rid[pj] = rid[m]; rfrom[pj] = rfrom[m];
+            long long nk = LLONG_MAX;
+            int nj = 0;
+            for (int r = 0; r < m; r++) {
+                const long long dk = ((std::llabs(jx - rx[r]) + std::llabs(jy - ry[r])) << sh) | rid[r];
+                if (dk < rk[r]) { rk[r] = dk; rfrom[r] = j; }
+                if (rk[r] < nk) { nk = rk[r]; nj = r; }
+            }
+            pj = nj;
         }
         // Steiner improvement: median stars on pairs of incident edges.
         auto D = [&](int a, int b) { return C.dist(nx[a], ny[a], nx[b], ny[b]); };
@@ -156,12 +175,25 @@ struct SubOut {
 };
 
 // Encode tree nodes with their pin layer or L (src/database_cuda.hpp:790-792).
+// The pins have distinct 2-D positions, so each node matches at most one.
 void encode_layers(const Grid& g, const std::vector<int>& pins, SubOut& s) {
     const int XY = g.X * g.Y;
+    if (pins.size() <= 16) {
+        for (auto& e : s.node) {
+            int pos = e % XY, lay = g.L;
+            for (int p : pins)
+                if (p % XY == pos) { lay = p / XY; break; }
+            e = pos + lay * XY;
+        }
+        return;
+    }
+    std::vector<std::pair<int, int>> byPos(pins.size());
+    for (size_t k = 0; k < pins.size(); k++) byPos[k] = {pins[k] % XY, pins[k] / XY};
+    std::sort(byPos.begin(), byPos.end());
     for (auto& e : s.node) {
-        int pos = e % XY, lay = g.L;
-        for (int p : pins)
-            if (p % XY == pos) { lay = p / XY; break; }
+        const int pos = e % XY;
+        auto it = std::lower_bound(byPos.begin(), byPos.end(), std::make_pair(pos, -1));
+        const int lay = it != byPos.end() && it->first == pos ? it->second : g.L;
         e = pos + lay * XY;
     }
 }
