@@ -61,6 +61,7 @@ def one_step(design, form, rrr, threads, profile=False):
     t0 = time.perf_counter()
     design.build(threads)
     t1 = time.perf_counter()
+    _progress("trees and pattern inputs built in %.3f s" % (t1 - t0))
     r = gpu.Router.from_design(design)
     if profile:
         r.set_profiling(True)
@@ -232,6 +233,7 @@ def main():
         print("bench.py measures one GPU only: the routing path runs on a single MI355X "
               "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
         sys.exit(2)
+    os.environ.setdefault("IGR_PROGRESS", "1")
     from instantgr_amd import gpu
     from instantgr_amd.host import Design
     copy_gbs = gpu.copy_probe(1 << 30, 20) if a.copy_probe else None

@@ -66,7 +66,7 @@ def _objs(srcs, subdir, compiler, flags, ext_out=".o"):
 
 def build_host(force=False):
     os.makedirs(LIB, exist_ok=True)
-    flags = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-result",
+    flags = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-result",
              "-I" + INCLUDE, "-I" + os.path.join(CSRC, "host")]
     objs = _objs(HOST_SRCS, "host", "g++", flags)
     out = os.path.join(LIB, "libigr_host.so")

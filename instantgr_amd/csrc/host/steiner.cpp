@@ -20,7 +20,7 @@
 #include <algorithm>
 #include <atomic>
 #include <climits>
-#include <functional>
+#include <limits>
 #include <numeric>
 #include <stdexcept>
 #include <thread>
@@ -31,9 +31,32 @@ namespace {
 
 constexpr int MAX_PIN_SIZE = 12;   // src/database_cuda.hpp:2111
 
+// Adjacency in compressed rows: the neighbours of node i are
+// nb[off[i] .. off[i + 1]).
+struct Adj {
+    std::vector<int> off, nb;
+};
+
+// Adjacency of n nodes from an edge list; each node lists its neighbours in
+// the order of the edges.
+void adj_from_edges(int n, const std::vector<std::pair<int, int>>& edges, Adj& a) {
+    a.off.assign(n + 1, 0);
+    for (auto& e : edges) {
+        a.off[e.first + 1]++;
+        a.off[e.second + 1]++;
+    }
+    for (int i = 0; i < n; i++) a.off[i + 1] += a.off[i];
+    a.nb.resize(a.off[n]);
+    std::vector<int> cur(a.off.begin(), a.off.end() - 1);
+    for (auto& e : edges) {
+        a.nb[cur[e.first]++] = e.second;
+        a.nb[cur[e.second]++] = e.first;
+    }
+}
+
 struct Tree2D {
-    std::vector<int> pos;                 // sorted distinct x*Y+y
-    std::vector<std::vector<int>> adj;    // Kruskal acceptance order
+    std::vector<int> pos;    // sorted distinct x*Y+y
+    Adj adj;                 // Kruskal acceptance order
 };
 
 struct Coords {
@@ -46,34 +69,99 @@ struct Coords {
 
 int median3(int a, int b, int c) { return std::max(std::min(a, b), std::min(std::max(a, b), c)); }
 
+// Disjoint sets with path halving (the roots are those of any other find).
+struct Dsu {
+    std::vector<int> p;
+    explicit Dsu(int n) : p(n) { std::iota(p.begin(), p.end(), 0); }
+    int find(int x) {
+        while (p[x] != x) {
+            p[x] = p[p[x]];
+            x = p[x];
+        }
+        return x;
+    }
+};
+
 // Normalise a node/edge set the way my_flute does.
 void normalise(const Coords& C, const std::vector<int>& nx, const std::vector<int>& ny,
                const std::vector<std::pair<int, int>>& edges, Tree2D& t) {
-    std::vector<int> p(nx.size());
-    for (size_t i = 0; i < nx.size(); i++) p[i] = nx[i] * C.Y + ny[i];
-    t.pos = p;
-    std::sort(t.pos.begin(), t.pos.end());
-    t.pos.erase(std::unique(t.pos.begin(), t.pos.end()), t.pos.end());
-    auto id = [&](int v) { return (int)(std::lower_bound(t.pos.begin(), t.pos.end(), v) - t.pos.begin()); };
+    const int n = (int)nx.size();
+    std::vector<std::pair<int, int>> byPos(n);
+    for (int i = 0; i < n; i++) byPos[i] = {nx[i] * C.Y + ny[i], i};
+    std::sort(byPos.begin(), byPos.end());
+    std::vector<int> id(n);
+    t.pos.clear();
+    for (int k = 0; k < n; k++) {
+        if (k == 0 || byPos[k].first != byPos[k - 1].first) t.pos.push_back(byPos[k].first);
+        id[byPos[k].second] = (int)t.pos.size() - 1;
+    }
     std::vector<std::tuple<long long, int, int>> es;
     es.reserve(edges.size());
     for (auto& e : edges) {
-        int u = id(p[e.first]), v = id(p[e.second]);
+        int u = id[e.first], v = id[e.second];
         if (u == v) continue;
         long long len = C.dist(nx[e.first], ny[e.first], nx[e.second], ny[e.second]);
         es.emplace_back(len, std::min(u, v), std::max(u, v));
     }
     std::sort(es.begin(), es.end());
-    std::vector<int> par(t.pos.size());
-    std::iota(par.begin(), par.end(), 0);
-    std::function<int(int)> find = [&](int x) { return par[x] == x ? x : par[x] = find(par[x]); };
-    t.adj.assign(t.pos.size(), {});
+    Dsu ds((int)t.pos.size());
+    std::vector<std::pair<int, int>> acc;
+    acc.reserve(es.size());
     for (auto& e : es) {
-        int u = std::get<1>(e), v = std::get<2>(e), pu = find(u), pv = find(v);
+        int u = std::get<1>(e), v = std::get<2>(e), pu = ds.find(u), pv = ds.find(v);
         if (pu == pv) continue;
-        t.adj[u].push_back(v);
-        t.adj[v].push_back(u);
-        par[pu] = pv;
+        acc.emplace_back(u, v);
+        ds.p[pu] = pv;
+    }
+    adj_from_edges((int)t.pos.size(), acc, t.adj);
+}
+
+// One Prim step over the nodes outside the tree, kept in ascending index
+// order: relax them against the node just added (j at jx, jy) and return the
+// position of the nearest, the first on ties, so the smallest index.
+template <class T>
+__attribute__((target_clones("avx2", "default")))
+int prim_step(const T* __restrict rx, const T* __restrict ry, T* __restrict rd, int* __restrict rfrom, int m, T jx,
+              T jy, int j) {
+    T mn = std::numeric_limits<T>::max();
+    for (int r = 0; r < m; r++) {
+        const T dx = jx - rx[r], dy = jy - ry[r];
+        const T d = (dx < 0 ? -dx : dx) + (dy < 0 ? -dy : dy);
+        const bool lt = d < rd[r];
+        rd[r] = lt ? d : rd[r];
+        rfrom[r] = lt ? j : rfrom[r];
+        mn = rd[r] < mn ? rd[r] : mn;
+    }
+    int r = 0;
+    while (rd[r] != mn) r++;
+    return r;
+}
+
+// Rectilinear minimum spanning tree by Prim from node 0, ties to the smallest
+// index; edges are appended to adj in the order they join the tree.
+template <class T>
+void prim(const Coords& C, const std::vector<int>& nx, const std::vector<int>& ny, std::vector<std::vector<int>>& adj) {
+    const int n = (int)nx.size();
+    int m = n - 1;
+    std::vector<T> rx(m), ry(m), rd(m, std::numeric_limits<T>::max());
+    std::vector<int> rid(m), rfrom(m, 0);
+    for (int r = 0; r < m; r++) {
+        rx[r] = (T)C.drx[nx[r + 1]];
+        ry[r] = (T)C.dry[ny[r + 1]];
+        rid[r] = r + 1;
+    }
+    int j = 0;
+    T jx = (T)C.drx[nx[0]], jy = (T)C.dry[ny[0]];
+    while (m > 0) {
+        const int p = prim_step<T>(rx.data(), ry.data(), rd.data(), rfrom.data(), m, jx, jy, j);
+        j = rid[p];
+        jx = rx[p];
+        jy = ry[p];
+        adj[rfrom[p]].push_back(j);
+        adj[j].push_back(rfrom[p]);
+        m--;
+        for (auto* v : {&rx, &ry, &rd}) std::copy(v->begin() + p + 1, v->begin() + m + 1, v->begin() + p);
+        for (auto* v : {&rid, &rfrom}) std::copy(v->begin() + p + 1, v->begin() + m + 1, v->begin() + p);
     }
 }
 
@@ -84,54 +172,25 @@ void steiner_tree(const Coords& C, const std::vector<int>& pts, Tree2D& t) {
     for (int i = 0; i < n; i++) { nx[i] = pts[i] / C.Y; ny[i] = pts[i] % C.Y; }
     std::vector<std::pair<int, int>> edges;
     if (n >= 2) {
-        // Prim, ties to the smallest index.  The nodes outside the tree sit
-        // in structure-of-arrays form; one pass per step relaxes them against
-        // the node just added and finds the next by the key (distance << sh)
-        // | index, whose minimum is the nearest node with the smallest index.
-        // The added node leaves by a swap with the last entry.
-        int sh = 1;
-        while ((1 << sh) <= n) sh++;
-        const long long span = C.drx.back() + C.dry.back();
-        if (span >= (1LL << (62 - sh))) throw std::runtime_error("Steiner tree: coordinate span too large");
-        std::vector<long long> rx(n - 1), ry(n - 1), rk(n - 1);
-        std::vector<int> rid(n - 1), rfrom(n - 1, 0);
-        const long long x0 = C.drx[nx[0]], y0 = C.dry[ny[0]];
-        int m = n - 1, pj = 0;
-        for (int r = 0; r < m; r++) {
-            const int k = r + 1;
-            rx[r] = C.drx[nx[k]];
-            ry[r] = C.dry[ny[k]];
-            rid[r] = k;
-            rk[r] = ((std::llabs(x0 - rx[r]) + std::llabs(y0 - ry[r])) << sh) | k;
-            if (rk[r] < rk[pj]) pj = r;
-        }
         std::vector<std::vector<int>> adj(n);
-        while (m > 0) {
-            const int j = rid[pj], fj = rfrom[pj];
-            const long long jx = rx[pj], jy = ry[pj];
-            adj[fj].push_back(j);
-            adj[j].push_back(fj);
-            m--;
-            rx[pj] = rx[m]; ry[pj] = ry[m]; rk[pj] = rk[m]; rid[pj] = rid[m]; rfrom[pj] = rfrom[m];
-            long long nk = LLONG_MAX;
-            int nj = 0;
-            for (int r = 0; r < m; r++) {
-                const long long dk = ((std::llabs(jx - rx[r]) + std::llabs(jy - ry[r])) << sh) | rid[r];
-                if (dk < rk[r]) { rk[r] = dk; rfrom[r] = j; }
-                if (rk[r] < nk) { nk = rk[r]; nj = r; }
-            }
-            pj = nj;
-        }
+        const long long span = C.drx.back() + C.dry.back();
+        if (span < INT_MAX / 2) prim<int>(C, nx, ny, adj);
+        else prim<long long>(C, nx, ny, adj);
         // Steiner improvement: median stars on pairs of incident edges.
         auto D = [&](int a, int b) { return C.dist(nx[a], ny[a], nx[b], ny[b]); };
         auto drop = [&](int a, int b) {
             auto& v = adj[a];
             v.erase(std::find(v.begin(), v.end(), b));
         };
+        // A node whose neighbourhood has not changed since it last found no
+        // gain finds none again, so the passes skip it (same result as
+        // re-evaluating every node in every pass).
+        std::vector<char> dirty(n, 1);
         bool changed = true;
         while (changed) {
             changed = false;
             for (int u = 0; u < (int)adj.size(); u++) {
+                if (!dirty[u]) continue;
                 for (;;) {
                     long long bestgain = 0;
                     int bv = -1, bw = -1, sx = 0, sy = 0;
@@ -145,7 +204,10 @@ void steiner_tree(const Coords& C, const std::vector<int>& pts, Tree2D& t) {
                                 - C.dist(nx[w], ny[w], mx, my);
                             if (g > bestgain) { bestgain = g; bv = v; bw = w; sx = mx; sy = my; }
                         }
-                    if (bestgain <= 0) break;
+                    if (bestgain <= 0) {
+                        dirty[u] = 0;
+                        break;
+                    }
                     drop(u, bv); drop(bv, u); drop(u, bw); drop(bw, u);
                     int s = -1;
                     for (int c : {u, bv, bw})
@@ -154,9 +216,11 @@ void steiner_tree(const Coords& C, const std::vector<int>& pts, Tree2D& t) {
                         s = (int)nx.size();
                         nx.push_back(sx); ny.push_back(sy);
                         adj.emplace_back();
+                        dirty.push_back(1);
                     }
                     for (int c : {u, bv, bw})
                         if (c != s) { adj[s].push_back(c); adj[c].push_back(s); }
+                    for (int c : {u, bv, bw, s}) dirty[c] = 1;
                     changed = true;
                 }
             }
@@ -171,7 +235,7 @@ void steiner_tree(const Coords& C, const std::vector<int>& pts, Tree2D& t) {
 struct SubOut {
     std::vector<int> pins;                // routing-layer pins
     std::vector<int> node;                // with layer (L = Steiner)
-    std::vector<std::vector<int>> adj;
+    Adj adj;
 };
 
 // Encode tree nodes with their pin layer or L (src/database_cuda.hpp:790-792).
@@ -211,8 +275,8 @@ void split_net(const Coords& C, const Grid& g, const std::vector<int>& pins, std
         out.emplace_back();
         SubOut& s = out.back();
         s.pins = pins;
-        s.node = t.pos;
-        s.adj = t.adj;
+        s.node = std::move(t.pos);
+        s.adj = std::move(t.adj);
         encode_layers(g, pins, s);
         return;
     }
@@ -223,15 +287,24 @@ void split_net(const Coords& C, const Grid& g, const std::vector<int>& pins, std
     const int n = (int)t.pos.size();
     std::vector<char> not_steiner(n);
     for (int i = 0; i < n; i++) not_steiner[i] = whole.node[i] < g.L * XY;
-    std::vector<int> sz(n, 1), par(n);
-    std::iota(par.begin(), par.end(), 0);
-    std::function<int(int)> find = [&](int x) { return par[x] == x ? x : par[x] = find(par[x]); };
+    std::vector<int> sz(n, 1);
+    Dsu ds(n);
+    std::vector<int>& par = ds.p;
+    auto find = [&](int x) { return ds.find(x); };
     std::vector<std::tuple<int, int, int>> edges;
     for (int i = 0; i < n; i++)
-        for (int j : t.adj[i])
+        for (int k = t.adj.off[i]; k < t.adj.off[i + 1]; k++) {
+            const int j = t.adj.nb[k];
             if (j < i) edges.emplace_back(j, i, not_steiner[i] + not_steiner[j]);
-    std::stable_sort(edges.begin(), edges.end(),
-                     [](const auto& a, const auto& b) { return std::get<2>(a) < std::get<2>(b); });
+        }
+    {   // stable order by the pin count of the edge's ends (0, 1 or 2)
+        std::vector<std::tuple<int, int, int>> by;
+        by.reserve(edges.size());
+        for (int key = 0; key <= 2; key++)
+            for (auto& e : edges)
+                if (std::get<2>(e) == key) by.push_back(e);
+        edges.swap(by);
+    }
     bool big_steiner = false;
     for (auto& e : edges) {
         int u = find(std::get<0>(e)), v = find(std::get<1>(e));
@@ -260,28 +333,28 @@ void split_net(const Coords& C, const Grid& g, const std::vector<int>& pins, std
         if (std::find(members[pu].begin(), members[pu].end(), v) == members[pu].end())
             members[pu].push_back(v);
     }
+    std::vector<int> local(n, -1);
+    std::vector<std::pair<int, int>> acc;
     for (int r = 0; r < n; r++) {
         if (find(r) != r || members[r].empty()) continue;
         SubOut s;
-        std::vector<int> local(n, -1);
         for (int m : members[r]) {
             local[m] = (int)s.node.size();
             s.node.push_back(whole.node[m]);
             if (whole.node[m] < g.L * XY) s.pins.push_back(whole.node[m]);
         }
-        s.adj.assign(s.node.size(), {});
-        std::vector<int> up(s.node.size());
-        std::iota(up.begin(), up.end(), 0);
-        std::function<int(int)> f2 = [&](int x) { return up[x] == x ? x : up[x] = f2(up[x]); };
+        Dsu up((int)s.node.size());
+        acc.clear();
         for (int ei : net_edge[r]) {
             int a = local[std::get<0>(edges[ei])], b = local[std::get<1>(edges[ei])];
             if (a < 0 || b < 0) continue;
-            int ra = f2(a), rb = f2(b);
+            int ra = up.find(a), rb = up.find(b);
             if (ra == rb) continue;
-            up[ra] = rb;
-            s.adj[a].push_back(b);
-            s.adj[b].push_back(a);
+            up.p[ra] = rb;
+            acc.emplace_back(a, b);
         }
+        adj_from_edges((int)s.node.size(), acc, s.adj);
+        for (int m : members[r]) local[m] = -1;
         if (s.node.size() >= 2) out.push_back(std::move(s));
     }
 }
@@ -324,7 +397,7 @@ void build_subnets(const Design& d, const NetPins& np, const Grid& g, Subnets& s
             for (const auto& so : per[n]) {
                 ps += (int64_t)so.pins.size();
                 ns += (int64_t)so.node.size();
-                for (const auto& v : so.adj) as += (int64_t)v.size();
+                as += (int64_t)so.adj.nb.size();
             }
             c_sub[n + 1] = (int64_t)per[n].size();
             c_pin[n + 1] = ps;
@@ -371,8 +444,9 @@ void build_subnets(const Design& d, const NetPins& np, const Grid& g, Subnets& s
                 for (size_t i = 0; i < so.node.size(); i++) {
                     s.node[nc] = so.node[i];
                     s.adj_off[nc] = ac;
-                    std::copy(so.adj[i].begin(), so.adj[i].end(), s.adj.begin() + ac);
-                    ac += (int64_t)so.adj[i].size();
+                    const int a0 = so.adj.off[i], a1 = so.adj.off[i + 1];
+                    std::copy(so.adj.nb.begin() + a0, so.adj.nb.begin() + a1, s.adj.begin() + ac);
+                    ac += a1 - a0;
                     nc++;
                 }
                 k++;
