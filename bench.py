@@ -43,6 +43,14 @@ def parse():
     ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--forms", default="plain,batched")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-profile", action="store_true", help="skip the untimed kernel-timing step")
+    ap.add_argument("--serve", metavar="RATES",
+                    help="serving benchmark instead: comma-separated Poisson request rates (req/s) against the "
+                         "routed configuration-1 design")
+    ap.add_argument("--serve-requests", type=int, default=200, help="requests per rate (serving benchmark)")
+    ap.add_argument("--autotune-nets", type=int, default=0,
+                    help="before timing, tune the batched launch shapes on a sample design of this many nets "
+                         "from the same generator (untimed)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (bandwidth reference, counter calibration kernel)")
     ap.add_argument("--summarize-stats", metavar="DIR",
@@ -56,13 +64,15 @@ def _progress(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def one_step(design, form, rrr, threads, profile=False):
+def one_step(design, form, rrr, threads, profile=False, tuning=None):
     from instantgr_amd import gpu
     t0 = time.perf_counter()
     design.build(threads)
     t1 = time.perf_counter()
     _progress("trees and pattern inputs built in %.3f s" % (t1 - t0))
     r = gpu.Router.from_design(design)
+    if tuning:
+        r.set_tuning(**tuning)
     if profile:
         r.set_profiling(True)
     st = r.run(design, form=form, rrr=rrr, threads=threads)
@@ -221,6 +231,44 @@ def summarize_pmc(fetch_root, write_root, copy_bytes=1 << 30):
     print("\n".join(lines))
 
 
+def serve_bench(a):
+    """Serving: the design stays routed on the device; Poisson reroute requests
+    of 1-5000 subnets from random 50x50 windows at each rate; p50/p99 latency,
+    nets/s, and the final score against a one-request-at-a-time replay."""
+    from instantgr_amd import gpu, serve
+    from instantgr_amd.host import Design
+    design = Design.preset(a.config, nets=a.nets).build(a.threads)
+    rates = [float(x) for x in a.serve.split(",") if x.strip()]
+    lines = []
+    with tempfile.TemporaryDirectory() as tmp:
+        for k, rate in enumerate(rates):
+            reqs = serve.make_requests(design, a.serve_requests, rate, window=50, kmax=5000, seed=100 + k)
+            r = gpu.Router.from_design(design)
+            r.run(design, form=gpu.FORM_BATCHED, rrr=a.rrr, threads=a.threads)
+            _progress("serving %.0f req/s: design routed" % rate)
+            res = serve.serve(r, design, reqs, threads=a.threads, max_tick_nets=20000, hold_ms=0)
+            p = os.path.join(tmp, "served.out")
+            design.write_routes(p, *r.routes_flat())
+            s_served = design.evaluate(p)["total"]
+            r.close()
+            r2 = gpu.Router.from_design(design)
+            r2.run(design, form=gpu.FORM_BATCHED, rrr=a.rrr, threads=a.threads)
+            serve.replay(r2, design, reqs, threads=a.threads)
+            design.write_routes(p, *r2.routes_flat())
+            s_replay = design.evaluate(p)["total"]
+            r2.close()
+            res.pop("results")
+            res.update({"rate": rate, "score": s_served, "replay_score": s_replay,
+                        "score_rel_diff": abs(s_served - s_replay) / max(1e-300, abs(s_replay))})
+            _progress("serving %.0f req/s: %s" % (rate, json.dumps(res)))
+            lines.append(res)
+    print(json.dumps({"metric": "reroute request latency and throughput, resident routed design",
+                      "config": {"workload": "configuration %d design, Poisson reroute requests of "
+                                             "floor(exp(U(0, ln 5000))) subnets from 50x50 windows" % a.config,
+                                 "requests_per_rate": a.serve_requests},
+                      "rates": lines}), flush=True)
+
+
 def main():
     a = parse()
     if a.summarize_stats:
@@ -236,35 +284,56 @@ def main():
     os.environ.setdefault("IGR_PROGRESS", "1")
     from instantgr_amd import gpu
     from instantgr_amd.host import Design
+    if a.serve:
+        serve_bench(a)
+        return
     copy_gbs = gpu.copy_probe(1 << 30, 20) if a.copy_probe else None
     t_gen = time.perf_counter()
     design = Design.preset(a.config, nets=a.nets)
     _progress("design generated in %.1f s" % (time.perf_counter() - t_gen))
+    tuning = None
+    if a.autotune_nets:
+        from instantgr_amd import autotune
+        t_at = time.perf_counter()
+        sample = Design.preset(a.config, nets=a.autotune_nets).build(a.threads)
+        tuning = autotune.tune(sample, rrr=1, threads=a.threads, log=_progress)["tuning"]
+        _progress("autotuned on %d nets in %.1f s: %s" % (a.autotune_nets, time.perf_counter() - t_at,
+                                                        json.dumps(tuning)))
     forms = [f.strip() for f in a.forms.split(",") if f.strip()]
     form_id = {"plain": gpu.FORM_PLAIN, "batched": gpu.FORM_BATCHED}
     results = {}
     routes = {}
     for f in forms:
         for w in range(a.warmup):
-            r, _st = one_step(design, form_id[f], a.rrr, a.threads)
+            r, _st = one_step(design, form_id[f], a.rrr, a.threads, tuning=tuning if f == "batched" else None)
             r.close()
             _progress("%s warmup %d: %.3f s" % (f, w, _st["step_s"]))
         times = []
         last = None
         for k in range(a.steps):
-            r, st = one_step(design, form_id[f], a.rrr, a.threads, profile=(k == a.steps - 1))
+            r, st = one_step(design, form_id[f], a.rrr, a.threads, tuning=tuning if f == "batched" else None)
             times.append(st["step_s"])
             _progress("%s step %d: %.3f s" % (f, k, st["step_s"]))
             if last is not None:
                 last[0].close()
             last = (r, st)
         r, st = last
+        routes[f] = r.routes_flat()
+        if not a.no_profile:
+            # Kernel timing (events on the stream, no graph) in one extra,
+            # untimed step.
+            r.close()
+            r, pst = one_step(design, form_id[f], a.rrr, a.threads, profile=True,
+                              tuning=tuning if f == "batched" else None)
+            _progress("%s profiled step: %.3f s" % (f, pst["step_s"]))
+            if not gpu.routes_equal(r.routes_flat(), routes[f]):
+                raise RuntimeError("profiled step routed differently")
         kt = r.kernel_times()
         kb = r.kernel_bytes()
-        routes[f] = r.routes_flat()
         results[f] = {"step_s": times, "stats": st, "kernels": kt, "bytes": kb}
         print("[bench] %s: %s" % (f, json.dumps({"step_s": times, "pattern_batches": st["pattern_batches"],
-                                                   "ripped": st["ripped"], "tree_s": st["tree_s"],
+                                                   "ripped": st["ripped"], "rrr_batches": st["rrr_batches"],
+                                                   "tree_s": st["tree_s"],
                                                    "pattern_s": st["pattern_s"],
                                                    "rrr_dag_build_s": st["rrr_dag_build_s"],
                                                    "rrr_route_s": st["rrr_route_s"],
@@ -289,9 +358,7 @@ def main():
             design.write_routes(p, off, data)
             scores[f] = design.evaluate(p)
     if len(forms) == 2:
-        (o1, d1), (o2, d2) = routes[forms[0]], routes[forms[1]]
-        same = all(np.array_equal(d1[o1[i]:o1[i] + d1[o1[i]]], d2[o2[i]:o2[i] + d2[o2[i]]])
-                   for i in range(len(o1) - 1))
+        same = gpu.routes_equal(routes[forms[0]], routes[forms[1]])
     head = "batched" if "batched" in results else forms[-1]
     step_ms = 1e3 * sum(results[head]["step_s"]) / len(results[head]["step_s"])
     value = nets_in / (step_ms * 1e-3)
@@ -303,7 +370,7 @@ def main():
         "config": {"workload": "configuration %d: %d metal layers, %dx%d gcells, %d nets; tree build + pattern "
                                "pass + %d rip-up-and-reroute passes" % (a.config, info.metal_layers, info.X, info.Y,
                                                                         nets_in, a.rrr),
-                   "subnets": int(info.subnets), "form": head},
+                   "subnets": int(info.subnets), "form": head, "tuning": tuning},
         "forms": {f: {"nets_per_s": nets_in / (sum(v["step_s"]) / len(v["step_s"])),
                       "ms_per_step": 1e3 * sum(v["step_s"]) / len(v["step_s"]),
                       "score": scores[f]} for f, v in results.items()},

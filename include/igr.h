@@ -174,6 +174,68 @@ int igr_kernel_times(igr_ctx* ctx, double* ms, int64_t* launches);
  * defined).  The per-unit figures are stated beside each launcher. */
 int igr_kernel_bytes(igr_ctx* ctx, double* bytes);
 
+/* Launch-shape choices of the batched form; 0 in a field keeps the built-in
+ * choice.  No counterpart in the reference (its launch shapes are fixed);
+ * the autotuner (instantgr_amd/autotune.py) measures and sets them.  Results
+ * do not depend on any of them. */
+typedef struct igr_tuning {
+    int graph;          /* 1: each pass's batch sequence captured and launched as one hipGraph; 2: stream launches */
+    int sched_group;    /* lanes per net in the scheduler's claim and viability kernels: 4, 8 or 16 */
+    int sched_chunk;    /* scheduler rounds enqueued between host checks, 1..256 */
+    int dag_order;      /* workgroup order of a DAG batch: 1 deepest DAG first, 2 routing order */
+    int dag_waves;      /* DAG kernel register budget: 2 (compiler's choice) or 4 waves per SIMD */
+} igr_tuning;
+int igr_set_tuning(igr_ctx* ctx, const igr_tuning* t);
+/* The values in force (built-in choices resolved, per-pass choices as 0). */
+int igr_get_tuning(igr_ctx* ctx, igr_tuning* t);
+
+/* Serving loop: a routed design kept resident on the device takes reroute
+ * requests (lists of subnets) as they arrive.  No counterpart in the
+ * reference, whose route() runs once per process (src/main.cpp:22-73); each
+ * request is a rip-up-and-reroute pass (Lshape_route_detour_wrap,
+ * src/Lshape_route_detour.hpp:726-1112) over its subnets, with the detour
+ * DAGs built from the congestion view at the start of its tick.
+ *
+ * A worker thread owns the context while the server is open (the caller
+ * makes no other call on that context until igr_server_close).  Each tick it
+ * admits waiting requests in arrival order within the budgets below, passing
+ * over any request whose footprint (current routes, DAG nodes and points of
+ * its subnets, on 16x16-gcell tiles) meets an earlier waiting or admitted
+ * request's footprint, and routes the admitted requests' subnets in one
+ * batched pass.  Requests admitted together therefore touch disjoint cells. */
+typedef struct igr_serve_opts {
+    int threads;              /* host threads for the DAG builder (0: 8) */
+    int64_t max_tick_nets;    /* subnets per tick (0: no limit; the first waiting request is always admitted) */
+    int64_t scratch_bytes;    /* estimated device scratch of the DAG tables per tick (0: no limit) */
+    int hold_ms;              /* a tick starts this long after its first request arrived, gathering
+                                 later arrivals into it (0: at once) */
+} igr_serve_opts;
+
+typedef struct igr_request_result {
+    int64_t id, nets;
+    double submit_s, start_s, done_s;   /* steady-clock seconds */
+    double latency_s;                   /* done - submit */
+    int tick;                           /* tick that routed the request */
+    int64_t tick_nets;                  /* subnets routed in that tick, all requests together */
+} igr_request_result;
+
+typedef struct igr_server_stats {
+    int64_t ticks, requests, nets, max_tick_nets;
+    double prep_s, route_s;             /* host preparation (view, DAGs, admission) and pass time, summed */
+} igr_server_stats;
+
+typedef struct igr_server igr_server;
+int igr_server_open(igr_ctx* ctx, struct igrh_design* design, const igr_serve_opts* opts, igr_server** out);
+/* Thread-safe; copies the list. */
+int igr_server_submit(igr_server* srv, const int* nets, int64_t n, int64_t* id);
+/* Blocks until the request is routed; a request is waited for once. */
+int igr_server_wait(igr_server* srv, int64_t id, igr_request_result* out);
+int igr_server_stats_get(igr_server* srv, igr_server_stats* out);
+/* Routes every waiting request, then stops the worker. */
+int igr_server_close(igr_server* srv);
+/* Message of the last failed igr_server_* call on the calling thread. */
+const char* igr_server_error(void);
+
 /* Current commit timestamp counter. */
 int igr_stamp(igr_ctx* ctx);
 

@@ -1,0 +1,117 @@
+"""Launch-shape autotuner of the batched form.
+
+The batched form has a few launch-shape choices that change speed and never
+results (include/igr.h, igr_tuning):
+
+  sched_group  lanes per net in the scheduler's claim and viability kernels
+  sched_chunk  scheduler rounds enqueued between host checks
+  graph        one hipGraph per pass for the batch sequence, or stream launches
+  dag_order    workgroup order of a DAG batch (deepest DAG first, or routing order)
+  dag_waves    register budget of the DAG kernel (2 or 4 waves per SIMD)
+
+`tune` measures them on a design (normally a reduced sample of the production
+design from the same generator) by coordinate descent: each field in turn takes
+the candidate value with the smallest measured time of the pass stage it
+affects, the others held at their best so far.  Every candidate run routes the
+same nets; the tuner checks that the routes do not change with the shape.
+
+The result is a plain dict that `save` / `load` keep as JSON, and that
+`apply` hands to a Router.
+"""
+import json
+import os
+import time
+
+from . import gpu
+
+CANDIDATES = {
+    "sched_group": [4, 8, 16],
+    "sched_chunk": [8, 16, 32, 64],
+    "graph": [1, 2],
+    "dag_order": [1, 2],
+    "dag_waves": [2, 4],
+}
+
+# Which measured stage each field affects.
+STAGE = {"sched_group": "schedule", "sched_chunk": "schedule", "graph": "route", "dag_order": "route",
+         "dag_waves": "route"}
+
+
+def _measure(design, tuning, rrr, threads):
+    """One pattern pass + rrr rip-up-and-reroute passes; stage times and routes."""
+    r = gpu.Router.from_design(design)
+    try:
+        r.set_tuning(**tuning)
+        t0 = time.perf_counter()
+        st = r.run(design, form=gpu.FORM_BATCHED, rrr=rrr, threads=threads)
+        wall = time.perf_counter() - t0
+        sched = st["pattern_schedule_s"] + sum(st["rrr_schedule_s"])
+        route = st["pattern_s"] - st["pattern_schedule_s"] + sum(st["rrr_route_s"])
+        return {"schedule": sched, "route": route, "wall": wall}, r.routes_flat()
+    finally:
+        r.close()
+
+
+def tune(design, rrr=1, threads=16, reps=1, candidates=None, log=None):
+    """Coordinate descent over the tuning fields on a built design.
+
+    Returns {"tuning": {...}, "trials": [...]}; every trial records the shape,
+    its stage times (best of reps) and whether its routes matched the first
+    trial's."""
+    cand = dict(CANDIDATES)
+    if candidates:
+        cand.update(candidates)
+    best = {k: 0 for k in cand}       # 0: built-in choice
+    trials = []
+    ref_routes = None
+    for field, values in cand.items():
+        scores = {}
+        for v in values:
+            shape = dict(best)
+            shape[field] = v
+            t_best = None
+            for _ in range(reps):
+                times, routes = _measure(design, shape, rrr, threads)
+                if ref_routes is None:
+                    ref_routes = routes
+                if not gpu.routes_equal(routes, ref_routes):
+                    raise RuntimeError("routes changed with launch shape %r" % (shape,))
+                if t_best is None or times[STAGE[field]] < t_best[STAGE[field]]:
+                    t_best = times
+            scores[v] = t_best[STAGE[field]]
+            trials.append({"shape": shape, "times": t_best, "routes_match": True})
+            if log:
+                log("autotune %s=%s: %s %.4f s" % (field, v, STAGE[field], scores[v]))
+        best[field] = min(values, key=lambda v: (scores[v], values.index(v)))
+    return {"tuning": best, "trials": trials}
+
+
+def key(design):
+    """Cache key of a design: grid shape and net count."""
+    i = design.info
+    return "L%d_X%d_Y%d_nets%d" % (i.L, i.X, i.Y, i.nets)
+
+
+def save(path, design_key, result):
+    data = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            data = json.load(f)
+    data[design_key] = result
+    with open(path, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+
+
+def load(path, design_key):
+    """The tuning stored for design_key, or None."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    res = data.get(design_key)
+    return res["tuning"] if res else None
+
+
+def apply(router, tuning):
+    if tuning:
+        router.set_tuning(**tuning)

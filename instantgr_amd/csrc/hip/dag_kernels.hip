@@ -344,7 +344,9 @@ inline unsigned blocks(int64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
 // ---------------------------------------------------------------------------
 // Batched form: one 64-lane workgroup routes one net's whole DAG, walking its
 // depth levels with workgroup barriers instead of one launch per level; all
-// nets of the batch run concurrently.  Per level the node DP runs one lane
+// nets of the batch run concurrently, taken from a list that puts the
+// deepest DAGs first (workgroups start in list order, so the longest chains
+// start first instead of forming the launch's tail).  Per level the node DP runs one lane
 // per node, then the parent-edge candidates run one lane per (edge, layer)
 // and go straight to the parent slots with 64-bit atomicMin (the minimum
 // over all candidates is the same as the reference's per-thread cache
@@ -385,23 +387,18 @@ __device__ void dp_node16(const DagArgs& a, int node, bool valid, int ncw) {
         cs = a.csum[node] - a.slot0;
     }
     const bool act = valid && s >= 1 && s <= L;
-    double ck[10];
-    uint32_t idk[10];
+    // per lane in LDS: the child keys (cost above id; padded with INIT_KEY),
+    // the best child choices so far (bp) and the current low's (cb), to leave
+    // registers for more waves per SIMD
+    __shared__ uint64_t s_key[WTPB][10], s_bp[WTPB][10], s_cb[WTPB][10];
+    uint64_t* key = s_key[lane];
+    uint64_t* bp = s_bp[lane];
+    uint64_t* cb = s_cb[lane];
 #pragma unroll
-    for (int k = 0; k < 10; k++) {
-        ck[k] = INF;
-        idk[k] = 0;
-        if (k < nc && act) {
-            const uint64_t key = a.packed[(cs + k) * ML + s];
-            ck[k] = unpack_cost(key);
-            idk[k] = (uint32_t)key;
-        }
-    }
-    float v[MAXL];
-#pragma unroll
-    for (int t = 0; t < MAXL; t++) v[t] = (valid && t < L) ? g.vcost[gidx(g, t, x, y)] : 0.0f;
+    for (int k = 0; k < 10; k++) key[k] = (k < nc && act) ? a.packed[(cs + k) * ML + s] : INIT_KEY;
+    // lane t of the group holds the via cut cost of layer t at the node
+    const float vl = (valid && s < L) ? g.vcost[gidx(g, s, x, y)] : 0.0f;
     double cst = INF;
-    uint64_t bp[10];
 #pragma unroll
     for (int k = 0; k < 10; k++) bp[k] = 0;
     // Lowest layer: group-uniform bound, the wave runs to its largest.
@@ -411,16 +408,18 @@ __device__ void dp_node16(const DagArgs& a, int node, bool valid, int ncw) {
         const bool run = act && low <= fixed_low;
         double via = 0;
 #pragma unroll
-        for (int t = 2; t <= MAXL; t++)
-            if (t > low && t <= s) via += (double)v[t - 2];
+        for (int t = 2; t <= MAXL; t++) {
+            const float vt = __shfl(vl, t - 2, 16);
+            if (t > low && t <= s) via += (double)vt;
+        }
         double cost = via;
-        uint64_t cb[10];
 #pragma unroll
         for (int k = 0; k < 10; k++) {
             cb[k] = 0;
             if (k >= ncw) continue;
-            double m = (run && s >= low && k < nc && ck[k] < INF) ? ck[k] : INF;
-            uint64_t b = (uint64_t)idk[k] * ML + s;
+            const double ck = unpack_cost(key[k]);
+            double m = (run && s >= low && k < nc && ck < INF) ? ck : INF;
+            uint64_t b = (uint64_t)(uint32_t)key[k] * ML + s;
 #pragma unroll
             for (int d = 1; d < 16; d <<= 1) {
                 const double om = __shfl_up(m, d, 16);
@@ -449,7 +448,7 @@ __device__ void dp_node16(const DagArgs& a, int node, bool valid, int ncw) {
 #pragma unroll
         for (int k = 0; k < 10; k++) {
             if (k >= ncw) continue;
-            const uint64_t nb = __shfl(bp[k], src, 16);
+            const uint64_t nb = s_bp[(lane & ~15) + src][k];
             if (run && s >= low && src != s) bp[k] = nb;
         }
         if (run && s >= low && src != s) cst = nc_cost;
@@ -536,11 +535,13 @@ __global__ void __launch_bounds__(TPB) k_dag_edges_pairs(DagArgs a, int64_t edge
     a.best_change[er * LL + q] = decide;
 }
 
-__global__ void __launch_bounds__(WTPB) k_dag_net(DagArgs a, NetLevels lv, const int* nets, int64_t net0, int64_t net1,
-                                                  const int64_t* node_off, const int64_t* route_slot, int* routes,
-                                                  int** rptr) {
-    const int64_t i = net0 + blockIdx.x;
-    if (i >= net1) return;
+// W: waves per SIMD the register allocation must allow (2 leaves the compiler
+// free; 4 caps it at 128 registers, spilling the rest).
+template <int W>
+__global__ void __launch_bounds__(WTPB) __attribute__((amdgpu_waves_per_eu(W, 8)))
+k_dag_net(DagArgs a, NetLevels lv, const int* nets, const int* list, const int64_t* node_off,
+          const int64_t* route_slot, int* routes, int** rptr) {
+    const int64_t i = list[blockIdx.x];
     const int lane = threadIdx.x;
     const GridDev& g = a.g;
     const int L = g.L, ML = L + 1;
@@ -658,8 +659,14 @@ void launch_dag_batch_wave(igr_ctx* c, const DagBatch& b, int stamp) {
     }
     NetLevels lv{D.nl_off.p, D.nlevels.p, D.lvl_node_off.p, D.lvl_edge_off.p, D.dnode.p, D.dedge.p};
     KScope ks(c, K_DAG_DP, b.bytes_dp);
-    k_dag_net<<<(unsigned)n, WTPB, 0, c->stream>>>(a, lv, D.nets.p, b.net0, b.net1, D.node_off.p, D.route_slot.p,
-                                                    c->nets.dag_routes.back()->p, c->nets.rptr.p);
+    if (c->tune.dag_waves == 4)
+        k_dag_net<4><<<(unsigned)n, WTPB, 0, c->stream>>>(a, lv, D.nets.p, D.glist.p + b.net0, D.node_off.p,
+                                                          D.route_slot.p, c->nets.dag_routes.back()->p,
+                                                          c->nets.rptr.p);
+    else
+        k_dag_net<2><<<(unsigned)n, WTPB, 0, c->stream>>>(a, lv, D.nets.p, D.glist.p + b.net0, D.node_off.p,
+                                                          D.route_slot.p, c->nets.dag_routes.back()->p,
+                                                          c->nets.rptr.p);
     HIP_OK(hipGetLastError());
 }
 

@@ -5,6 +5,9 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <functional>
@@ -32,6 +35,18 @@ static int guard(F&& f) {
         g_err = e.what();
         return IGR_EHIP;
     }
+}
+
+void igr_dev::progress(const char* fmt, ...) {
+    static const bool on = std::getenv("IGR_PROGRESS") != nullptr;
+    if (!on) return;
+    va_list ap;
+    va_start(ap, fmt);
+    std::fputs("[igr] ", stderr);
+    std::vfprintf(stderr, fmt, ap);
+    std::fputc('\n', stderr);
+    std::fflush(stderr);
+    va_end(ap);
 }
 
 static double now_s() {
@@ -194,6 +209,7 @@ int igr_set_nets(igr_ctx* c, const igr_net_desc* d) {
         std::vector<int> init(N.h_route_off[n], 0);
         for (int64_t i = 0; i < n; i++) init[N.h_route_off[i]] = 1;
         N.routes.upload(init.data(), init.size(), s);
+        N.h_rbuf.assign(n, -1);
         N.clear_dag_routes();
         N.h_rptr.resize(n);
         N.h_rcap.resize(n);
@@ -274,6 +290,7 @@ void reset_pattern_ptrs(igr_ctx* c, const int* nets, int64_t n) {
     for (int64_t i = 0; i < n; i++) {
         int* p = N.routes.p + N.h_route_off[nets[i]];
         if (N.h_rptr[nets[i]] != p) {
+            N.release(nets[i]);
             N.h_rptr[nets[i]] = p;
             N.h_rcap[nets[i]] = N.h_route_off[nets[i] + 1] - N.h_route_off[nets[i]];
             dirty = true;
@@ -318,6 +335,7 @@ int igr_pattern_route(igr_ctx* c, const int* order, int64_t n, int form, igr_pas
         int nb = form == IGR_FORM_PLAIN ? run_schedule(c, order, n, 0, batch_of.data())
                                         : run_schedule_batched(c, order, n, 0, batch_of.data());
         double t1 = now_s();
+        progress("pattern pass: %d batches scheduled in %.3f s", nb, t1 - t0);
         // Batches in routing order: the scheduler's last batch first; inside a
         // batch, the scheduler's index order (src/Lshape_route.hpp:175-183).
         std::vector<int64_t> cnt(nb + 1, 0);
@@ -522,6 +540,7 @@ void dag_pass_batched(igr_ctx* c, const int* nets, int64_t n, int mode, const ig
     helper.join();
     if (herr) std::rethrow_exception(herr);
     if (bad_dag.load()) throw std::invalid_argument("malformed DAG (edge offsets, depths or slot counts)");
+    progress("DAG pass: %d batches scheduled in %.3f s", nb, now_s() - t0);
     const double t1 = now_s();
 
     // Routing order: batch nb-1 first, inside a batch decreasing list
@@ -548,8 +567,19 @@ void dag_pass_batched(igr_ctx* c, const int* nets, int64_t n, int mode, const ig
     const int64_t NS = sbase[n], NL = lbase[n];
     std::vector<DagBatch> batches(nb);
     int64_t mx_nodes = 0, mx_edges = 0, mx_slots = 0;
+    // Per batch, the routed nets in launch order of the workgroup-per-net
+    // kernel: deepest DAG first, then most nodes (dag_order 1), or routing
+    // order (2).
+    const bool deep_first = (c->tune.dag_order ? c->tune.dag_order : DEF_DAG_ORDER) == 1;
+    std::vector<int> glist(n);
+    std::vector<std::pair<int64_t, int64_t>> gl;   // (sort key, routed net)
     for (int b = 0; b < nb; b++) {
         DagBatch& B = batches[b];
+        gl.clear();
+        for (int64_t r = cntb[b]; r < cntb[b + 1]; r++)
+            gl.emplace_back(deep_first ? -((lbase[r + 1] - lbase[r]) << 32 | (nbase[r + 1] - nbase[r])) : 0, r);
+        if (deep_first) std::sort(gl.begin(), gl.end());
+        for (size_t k = 0; k < gl.size(); k++) glist[cntb[b] + k] = (int)gl[k].second;
         B.net0 = cntb[b];
         B.net1 = cntb[b + 1];
         B.node0 = nbase[B.net0];
@@ -570,6 +600,7 @@ void dag_pass_batched(igr_ctx* c, const int* nets, int64_t n, int mode, const ig
         mx_edges = std::max(mx_edges, B.edge1 - B.edge0);
         mx_slots = std::max(mx_slots, B.slot1 - B.slot0);
     }
+    D.glist.upload(glist.data(), n, s);
     D.nets.upload(rnets.data(), n, s);
     D.rorder.upload(rorder.data(), n, s);
     D.node_off.upload(nbase.data(), n + 1, s);
@@ -628,6 +659,9 @@ void dag_pass_batched(igr_ctx* c, const int* nets, int64_t n, int mode, const ig
     // (rip-up, then commit, per batch).
     {
         BatchedState bs(c);
+        // The whole sequence is recorded as one graph unless kernel timing
+        // (events on the stream) is on.
+        StreamGraph sg(s, !c->prof.on && (c->tune.graph ? c->tune.graph : DEF_GRAPH) == 1);
         int s1 = ++c->stamp;
         launch_rip_scatter(c, D.nets.p + batches[0].net0, batches[0].net1 - batches[0].net0);
         launch_track_fused(c, 0, s1, true);
@@ -643,13 +677,11 @@ void dag_pass_batched(igr_ctx* c, const int* nets, int64_t n, int mode, const ig
                 launch_track_fused(c, s2, 0, false);
             }
         }
+        sg.launch();
         bs.check();
     }
     HIP_OK(hipStreamSynchronize(s));
-    for (int64_t r = 0; r < n; r++) {
-        N.h_rptr[rnets[r]] = rbase + route_slot[r];
-        N.h_rcap[rnets[r]] = route_slot[r + 1] - route_slot[r];
-    }
+    N.adopt(rnets.data(), route_slot.data(), n, rbase);
     const double t2 = now_s();
     c->last_dag_prep_s = tp - t1;
     c->last_dag_prep_parts[0] = t_meta - t1;     // routing-order bases, batch bounds, small uploads
@@ -702,6 +734,7 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
         std::vector<int> batch_of(n);
         int nb = run_schedule(c, nets, n, mode, batch_of.data(), D.pt_off.p, D.pt.p);
         double t1 = now_s();
+        progress("DAG pass: %d batches scheduled in %.3f s", nb, t1 - t0);
 
         // Routing order: batch nb-1 first, inside a batch decreasing list
         // position (the scheduler's index order).
@@ -878,7 +911,12 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
         int* rbase = N.dag_routes.back()->p;
         double tp = now_s();
 
+        double t_note = tp;
         for (const DagBatch& B : batches) {
+            if (now_s() - t_note > 20) {
+                t_note = now_s();
+                progress("DAG pass: batch %lld of %d routed", (long long)(&B - batches.data()), nb);
+            }
             const int* bn = D.nets.p + B.net0;
             const int64_t cnt = B.net1 - B.net0;
             const int s1 = ++c->stamp;
@@ -894,10 +932,7 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
             HIP_OK(hipStreamSynchronize(s));
         }
         HIP_OK(hipStreamSynchronize(s));
-        for (int64_t r = 0; r < n; r++) {
-            N.h_rptr[rnets[r]] = rbase + route_slot[r];
-            N.h_rcap[rnets[r]] = route_slot[r + 1] - route_slot[r];
-        }
+        N.adopt(rnets.data(), route_slot.data(), n, rbase);
         double t2 = now_s();
         c->last_dag_prep_s = tp - t1;
         c->last_dag_prep_parts[0] = t_fill - t1;
@@ -931,6 +966,31 @@ int igr_congestion_view(igr_ctx* c, float* xsum, float* ysum) {
 }
 
 int igr_stamp(igr_ctx* c) { return c->stamp; }
+
+int igr_set_tuning(igr_ctx* c, const igr_tuning* t) {
+    return guard([&]() {
+        if (!t) throw std::invalid_argument("null tuning");
+        if (t->graph < 0 || t->graph > 2) throw std::invalid_argument("graph: 0, 1 or 2");
+        if (t->sched_group != 0 && t->sched_group != 4 && t->sched_group != 8 && t->sched_group != 16)
+            throw std::invalid_argument("sched_group: 0, 4, 8 or 16");
+        if (t->sched_chunk < 0 || t->sched_chunk > 256) throw std::invalid_argument("sched_chunk: 0..256");
+        if (t->dag_order < 0 || t->dag_order > 2) throw std::invalid_argument("dag_order: 0, 1 or 2");
+        if (t->dag_waves != 0 && t->dag_waves != 2 && t->dag_waves != 4)
+            throw std::invalid_argument("dag_waves: 0, 2 or 4");
+        c->tune = *t;
+    });
+}
+
+int igr_get_tuning(igr_ctx* c, igr_tuning* t) {
+    return guard([&]() {
+        if (!t) throw std::invalid_argument("null tuning");
+        *t = c->tune;
+        if (!t->graph) t->graph = DEF_GRAPH;
+        if (!t->sched_chunk) t->sched_chunk = DEF_SCHED_CHUNK;
+        if (!t->dag_order) t->dag_order = DEF_DAG_ORDER;
+        if (!t->dag_waves) t->dag_waves = DEF_DAG_WAVES;
+    });
+}
 
 int igr_set_profiling(igr_ctx* c, int on) {
     return guard([&]() {

@@ -63,19 +63,52 @@ struct NetsDev {
     std::vector<int*> h_rptr;
     std::vector<int64_t> h_rcap;      // capacity of each net's current route slot
     std::vector<double> h_lbytes;     // algorithmic bytes of each net's L-shape DP
-    std::vector<DevBuf<int>*> dag_routes;
+    std::vector<DevBuf<int>*> dag_routes;   // freed (null) once no net's route lives there
+    std::vector<int64_t> dag_live;          // nets whose route lives in dag_routes[k]
+    std::vector<int> h_rbuf;                // per net: its dag_routes buffer, -1 = pattern slot
     int max_nodes = 0;
     ~NetsDev() { clear_dag_routes(); }
     void clear_dag_routes() {
         for (auto* b : dag_routes) delete b;
         dag_routes.clear();
+        dag_live.clear();
+        std::fill(h_rbuf.begin(), h_rbuf.end(), -1);
+    }
+    // Net 'net' leaves its rip-up-and-reroute buffer; an emptied buffer is
+    // freed (the caller has synchronised with every kernel that used it).
+    void release(int net) {
+        const int k = h_rbuf[net];
+        if (k < 0) return;
+        h_rbuf[net] = -1;
+        if (--dag_live[k] == 0) {
+            delete dag_routes[k];
+            dag_routes[k] = nullptr;
+        }
+    }
+    // The listed nets' routes now live in the newest buffer at rbase + slot[r].
+    void adopt(const int* rnets, const int64_t* slot, int64_t n, int* rbase) {
+        const int k = (int)dag_routes.size() - 1;
+        dag_live.resize(dag_routes.size(), 0);
+        for (int64_t r = 0; r < n; r++) {
+            const int net = rnets[r];
+            release(net);
+            h_rptr[net] = rbase + slot[r];
+            h_rcap[net] = slot[r + 1] - slot[r];
+            h_rbuf[net] = k;
+            dag_live[k]++;
+        }
+        if (dag_live[k] == 0) {
+            delete dag_routes[k];
+            dag_routes[k] = nullptr;
+        }
     }
 };
 
 struct Sched {
     DevBuf<int> map, batch, status, viable, ids, pids, flags;
-    // batched form: per-cell net lists, their heads and the round-0 map
-    DevBuf<int> base, cnt, c_net, ctrl;
+    // batched form: per-cell net lists, their heads and the round-0 map; the
+    // compacted unassigned / active net lists
+    DevBuf<int> base, cnt, c_net, ctrl, lists;
     DevBuf<int64_t> head, c_off, bsum;
 };
 
@@ -97,6 +130,8 @@ struct DagDev {
     // per-routed-net bases and the level cursors of the device layout.
     DevBuf<int64_t> r_node_off, r_par_off, ebase, sbase, lbase, cur_n, cur_e;
     DevBuf<int> r_nodes, r_par, r_slot, r_child_num, r_depth, rorder, err;
+    // Batched form: the routed nets of each batch in launch order.
+    DevBuf<int> glist;
 };
 
 // Arguments of the device DAG layout (one workgroup per routed net).
@@ -198,6 +233,7 @@ struct igr_ctx {
     igr_dev::Scratch scratch;
     igr_dev::DagDev dag;
     int stamp = 0;
+    igr_tuning tune{0, 0, 0, 0, 0};   // launch-shape overrides (0 = built-in)
     double last_dag_prep_s = 0;   // host preparation time of the last DAG pass
     double last_dag_prep_parts[4] = {0, 0, 0, 0};   // fill, batch metadata, upload, allocation
     int64_t cells() const { return (int64_t)g.L * g.X * g.Y; }
@@ -224,6 +260,45 @@ struct KScope {
                 c->prof.cls.push_back(k);
                 c->prof.rec.emplace_back(a, b);
             }
+        }
+    }
+};
+
+// IGR_PROGRESS set in the environment: one line per stage on stderr, for
+// long runs watched from outside.
+void progress(const char* fmt, ...);
+
+// Built-in launch-shape choices (igr_tuning fields left at 0).
+constexpr int DEF_GRAPH = 1;
+constexpr int DEF_SCHED_CHUNK = 16;
+constexpr int DEF_DAG_ORDER = 1;
+constexpr int DEF_DAG_WAVES = 2;
+
+// Records the work enqueued on a stream between construction and launch()
+// as one graph, then launches it; inactive, the work runs as it is enqueued.
+// A capture still open when the scope unwinds is ended and dropped.
+struct StreamGraph {
+    hipStream_t s;
+    bool on;
+    explicit StreamGraph(hipStream_t s_, bool on_) : s(s_), on(on_) {
+        if (on) HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    }
+    void launch() {
+        if (!on) return;
+        on = false;
+        hipGraph_t g = nullptr;
+        HIP_OK(hipStreamEndCapture(s, &g));
+        hipGraphExec_t x = nullptr;
+        hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        if (e == hipSuccess) e = hipGraphLaunch(x, s);
+        if (x) (void)hipGraphExecDestroy(x);
+        (void)hipGraphDestroy(g);
+        HIP_OK(e);
+    }
+    ~StreamGraph() {
+        if (on) {
+            hipGraph_t g = nullptr;
+            if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
         }
     }
 };

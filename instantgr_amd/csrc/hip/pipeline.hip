@@ -16,8 +16,6 @@
 
 #include <algorithm>
 #include <chrono>
-#include <cstdio>
-#include <cstdlib>
 #include <numeric>
 
 using namespace igr_dev;
@@ -25,15 +23,6 @@ using namespace igr_dev;
 namespace {
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-// IGR_PROGRESS set in the environment: one line per stage on stderr, for
-// long runs watched from outside.
-void progress(const char* stage, int it, double s) {
-    static const bool on = std::getenv("IGR_PROGRESS") != nullptr;
-    if (!on) return;
-    if (it >= 0) std::fprintf(stderr, "[igr] %s %d: %.3f s\n", stage, it, s);
-    else std::fprintf(stderr, "[igr] %s: %.3f s\n", stage, s);
-    std::fflush(stderr);
 }
 } // namespace
 
@@ -58,7 +47,7 @@ int igr_run(igr_ctx* c, igrh_design* h, const igr_run_opts* o, igr_run_stats* st
     S.pattern_s = t2 - t1;
     S.pattern_batches = ps.batches;
     S.pattern_schedule_s = ps.schedule_s;
-    progress("pattern pass", -1, S.pattern_s);
+    progress("pattern pass: %.3f s", S.pattern_s);
     const int mode = pi.n > 20000000 ? 2 : (pi.n > 5000000 ? 1 : 0);
     const int64_t XYc = (int64_t)c->g.X * c->g.Y;
     std::vector<float> xs(XYc), ys(XYc);
@@ -92,7 +81,8 @@ int igr_run(igr_ctx* c, igrh_design* h, const igr_run_opts* o, igr_run_stats* st
         S.rrr_batches[it] = ps.batches;
         S.rrr_dag_nodes[it] = D.node_off.empty() ? 0 : D.node_off.back();
         S.iterations = it + 1;
-        progress("rip-up-and-reroute pass", it, e - a);
+        progress("rip-up-and-reroute pass %d: %.3f s (view %.3f, DAG build %.3f, %lld nets)", it, e - a, b - a,
+                 cc - b, (long long)nets.size());
     }
     S.total_s = now_s() - t0;
     S.subnets = pi.n;

@@ -12,12 +12,18 @@
 //  * In later rounds a point claim can only change a cell the previous
 //    round's clear emptied (values otherwise only grow within a batch), so a
 //    net claims a point only where the map holds a lower value.
+//  * The round kernels walk compacted net lists instead of every net: the
+//    unassigned nets (rebuilt at each batch end) and the nets still active in
+//    the batch (rebuilt after each round that committed, dropping committed
+//    and blocked nets).  The list order is free: every decision is an atomic
+//    maximum or a test against the map.
 //  * The round and batch control lives in a device word block; the host
 //    enqueues rounds in chunks and reads back one flag per chunk.  Kernels of
 //    rounds after the end return at once.
 #include "igr_ctx.hpp"
 
 #include <algorithm>
+#include <chrono>
 
 namespace igr_dev {
 
@@ -25,10 +31,12 @@ namespace {
 
 constexpr int TPB = 256;
 constexpr int ROUNDS = 30;        // rounds per batch (src/database_cuda.hpp:612)
-constexpr int CHUNK = 16;         // rounds enqueued per host check
 
-// control state words (two slots of C_WORDS, then the placed-net counter)
-enum { C_DONE = 0, C_CUR, C_ROUND, C_COMMIT, C_HAVE, C_NVIABLE, C_WORDS };
+// control state words (two slots of C_WORDS, then the placed-net counter).
+// C_NA / C_AL: length and buffer of the batch's active list (C_NA < 0: the
+// unassigned list itself); C_NU / C_UL: the same for the unassigned list.
+enum { C_DONE = 0, C_CUR, C_ROUND, C_COMMIT, C_HAVE, C_NVIABLE, C_NA, C_AL, C_NU, C_UL, C_WORDS };
+constexpr int NLISTS = 4;         // list buffers of n entries each
 
 struct BArgs {
     int64_t n;
@@ -48,8 +56,15 @@ struct BArgs {
     int* batch;
     int* status;                  // 1 active in this batch, 2 blocked, 0 assigned earlier
     int* ctrl;
+    int* lists;                   // NLISTS buffers of n net indices
     int X, Y, sparsity;
 };
+
+// The batch's active list of the round whose state is st.
+__device__ __forceinline__ const int* act_list(const BArgs& a, const int* st, int64_t& na) {
+    na = st[C_NA] < 0 ? st[C_NU] : st[C_NA];
+    return a.lists + (int64_t)st[C_AL] * a.n;
+}
 
 __device__ __forceinline__ bool active(const BArgs& a, int64_t i) { return a.status[i] == 1 && a.batch[i] == -1; }
 
@@ -185,9 +200,12 @@ template <int G>
 __global__ void __launch_bounds__(TPB) k_b_claim(BArgs a, int64_t g) {
     const int* st = a.ctrl + (g & 1) * C_WORDS;
     if (st[C_DONE] || st[C_ROUND] == 0) return;
+    int64_t na;
+    const int* al = act_list(a, st, na);
     const int64_t nth = (int64_t)gridDim.x * TPB / G;
     const int lg = threadIdx.x & (G - 1);
-    for (int64_t i = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G; i < a.n; i += nth) {
+    for (int64_t t = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G; t < na; t += nth) {
+        const int64_t i = al[t];
         if (!active(a, i)) continue;
         const int64_t k1 = a.pt_off[a.pids[i] + 1];
         for (int64_t k = a.pt_off[a.pids[i]] + lg; k < k1; k += G) {
@@ -202,12 +220,21 @@ template <int G>
 __global__ void __launch_bounds__(TPB) k_b_viable(BArgs a, int64_t g, int* vlist) {
     int* st = a.ctrl + (g & 1) * C_WORDS;
     if (st[C_DONE]) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the successor lists' counters, filled by this round's last kernel
+        int* nx = a.ctrl + ((g + 1) & 1) * C_WORDS;
+        nx[C_NA] = 0;
+        nx[C_NU] = 0;
+    }
+    int64_t na;
+    const int* al = act_list(a, st, na);
     const int64_t nth = (int64_t)gridDim.x * TPB / G;
     const int lg = threadIdx.x & (G - 1);
-    const int64_t rounds = (a.n + nth - 1) / nth;   // the same for every lane of the wave
+    const int64_t rounds = (na + nth - 1) / nth;   // the same for every lane of the wave
     for (int64_t r = 0; r < rounds; r++) {
-        const int64_t i = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G + r * nth;
-        const bool live = i < a.n && active(a, i);
+        const int64_t t = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G + r * nth;
+        const int64_t i = t < na ? al[t] : 0;
+        const bool live = t < na && active(a, i);
         const int64_t k0 = live ? a.pt_off[a.pids[i]] : 0, k1 = live ? a.pt_off[a.pids[i] + 1] : 0;
         bool fail = !live;
         for (int64_t k = k0 + lg;; k += G) {
@@ -272,12 +299,15 @@ template <int G>
 __global__ void __launch_bounds__(TPB) k_b_revert(BArgs a, int64_t g) {
     const int* st = a.ctrl + (g & 1) * C_WORDS;
     if (st[C_DONE] || !st[C_COMMIT]) return;
+    int64_t na;
+    const int* al = act_list(a, st, na);
     const int64_t nth = (int64_t)gridDim.x * TPB / G;
     const int lg = threadIdx.x & (G - 1);
-    const int64_t rounds = (a.n + nth - 1) / nth;
+    const int64_t rounds = (na + nth - 1) / nth;
     for (int64_t r = 0; r < rounds; r++) {
-        const int64_t i = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G + r * nth;
-        const bool live = i < a.n && active(a, i);
+        const int64_t t = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G + r * nth;
+        const int64_t i = t < na ? al[t] : 0;
+        const bool live = t < na && active(a, i);
         const int64_t k0 = live ? a.pt_off[a.pids[i]] : 0, k1 = live ? a.pt_off[a.pids[i] + 1] : 0;
         bool hit = false;
         for (int64_t k = k0 + lg;; k += G) {
@@ -296,8 +326,9 @@ __global__ void __launch_bounds__(TPB) k_b_revert(BArgs a, int64_t g) {
 // 6. end of the round.  The batch goes on after a commit below the round
 //    limit: cells of blocked nets are released.  Otherwise the batch ends:
 //    each cell's list moves past assigned nets, the next batch starts from
-//    the kept map with every unassigned net active.  Thread 0 writes the
-//    successor control state.
+//    the kept map with every unassigned net active.  The successor lists go
+//    to a buffer neither current list uses.  Thread 0 writes the successor
+//    control state.
 __global__ void __launch_bounds__(TPB) k_b_clear_end(BArgs a, int64_t g, int64_t cells) {
     const int* st = a.ctrl + (g & 1) * C_WORDS;
     int* nx = a.ctrl + ((g + 1) & 1) * C_WORDS;
@@ -306,6 +337,9 @@ __global__ void __launch_bounds__(TPB) k_b_clear_end(BArgs a, int64_t g, int64_t
     const bool commit = st[C_COMMIT] != 0;
     const int round = st[C_ROUND];
     const bool end = !commit || round + 1 >= ROUNDS;
+    int fresh = 0;   // the first list buffer neither current list uses
+    while (fresh == st[C_AL] || fresh == st[C_UL]) fresh++;
+    int* nl = a.lists + (int64_t)fresh * a.n;
     if (!done) {
         if (end) {
             if (t < cells) {
@@ -317,10 +351,23 @@ __global__ void __launch_bounds__(TPB) k_b_clear_end(BArgs a, int64_t g, int64_t
                 a.base[t] = b;
                 a.map[t] = b;
             }
-            if (t < a.n) a.status[t] = a.batch[t] == -1 ? 1 : 0;
-        } else if (t < cells) {
-            const int m = a.map[t];
-            if (m >= 0 && a.status[m] == 2) a.map[t] = -1;
+            if (t < st[C_NU]) {
+                const int i = a.lists[(int64_t)st[C_UL] * a.n + t];
+                const bool left = a.batch[i] == -1;
+                a.status[i] = left ? 1 : 0;
+                if (left) nl[atomicAdd(&nx[C_NU], 1)] = i;
+            }
+        } else {
+            if (t < cells) {
+                const int m = a.map[t];
+                if (m >= 0 && a.status[m] == 2) a.map[t] = -1;
+            }
+            int64_t na;
+            const int* al = act_list(a, st, na);
+            if (t < na) {
+                const int i = al[t];
+                if (active(a, i)) nl[atomicAdd(&nx[C_NA], 1)] = i;
+            }
         }
     }
     if (t == 0) {
@@ -343,12 +390,38 @@ __global__ void __launch_bounds__(TPB) k_b_clear_end(BArgs a, int64_t g, int64_t
         nx[C_HAVE] = have;
         nx[C_COMMIT] = 0;
         nx[C_NVIABLE] = 0;
+        if (done) {
+            nx[C_NA] = st[C_NA];
+            nx[C_AL] = st[C_AL];
+            nx[C_NU] = st[C_NU];
+            nx[C_UL] = st[C_UL];
+        } else if (end) {
+            nx[C_NA] = -1;           // the new unassigned list is the active list
+            nx[C_AL] = fresh;
+            nx[C_UL] = fresh;
+        } else {
+            nx[C_AL] = fresh;        // C_NA counted above
+            nx[C_NU] = st[C_NU];
+            nx[C_UL] = st[C_UL];
+        }
     }
 }
 
 __global__ void k_fill_b(int* p, int64_t n, int v) {
     const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// Start state: every net unassigned and active, listed in buffer 0.
+__global__ void k_start(BArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i < a.n) a.lists[i] = (int)i;
+    if (i == 0) {
+        a.ctrl[C_NA] = -1;
+        a.ctrl[C_AL] = 0;
+        a.ctrl[C_NU] = (int)a.n;
+        a.ctrl[C_UL] = 0;
+    }
 }
 
 inline unsigned blocks(int64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
@@ -380,10 +453,11 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
     S.batch.alloc(n);
     S.status.alloc(n);
     S.viable.alloc(n);
+    S.lists.alloc(NLISTS * n);
     S.ctrl.zero(2 * C_WORDS + 1, st);
     BArgs a{n, S.ids.p, d_pt_off ? S.pids.p : S.ids.p, d_pt_off ? d_pt_off : N.pt_off.p, d_pt_off ? d_pt : N.pt.p,
             N.hs_off.p, N.hs.p, N.vs_off.p, N.vs.p, S.map.p, S.base.p, S.c_off.p, nullptr, S.head.p,
-            S.batch.p, S.status.p, S.ctrl.p, c->g.X, c->g.Y,
+            S.batch.p, S.status.p, S.ctrl.p, S.lists.p, c->g.X, c->g.Y,
             (mode == 1 || mode == 2) ? 2 : 15};
     // Per-cell lists of the nets' point cells, highest priority first.
     k_cell_count<<<blocks(n), TPB, 0, st>>>(a, S.cnt.p);
@@ -403,11 +477,13 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
     k_cell_sort<<<blocks(XYc), TPB, 0, st>>>(a, XYc);
     k_fill_b<<<blocks(n), TPB, 0, st>>>(S.batch.p, n, -1);
     k_fill_b<<<blocks(n), TPB, 0, st>>>(S.status.p, n, 1);
+    k_start<<<blocks(n), TPB, 0, st>>>(a);
     HIP_OK(hipGetLastError());
     // Lanes per net: about half the average point count, at most 16.
     const int64_t entries_total = entries;
     const double avg = (double)entries_total / (double)n;
-    const int G = avg <= 6 ? 4 : avg <= 16 ? 8 : 16;
+    const int G = c->tune.sched_group ? c->tune.sched_group : avg <= 6 ? 4 : avg <= 16 ? 8 : 16;
+    const int CHUNK = c->tune.sched_chunk ? c->tune.sched_chunk : DEF_SCHED_CHUNK;
     const unsigned ggrid = (unsigned)std::min<int64_t>(2048, blocks(n * G));
     const unsigned wgrid = 256;   // one wave per viable net
     const unsigned gm = blocks(std::max(XYc, n));
@@ -415,9 +491,17 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
     HIP_OK(hipHostMalloc(&h_ctrl, sizeof(int) * (2 * C_WORDS + 1)));
     const int64_t max_rounds = (int64_t)ROUNDS * (n + 1) + CHUNK;
     int64_t g = 0;
+    const auto t_start = std::chrono::steady_clock::now();
+    auto t_note = t_start;
     try {
         for (;;) {
             if (g > max_rounds) throw std::runtime_error("scheduler did not finish");
+            const auto t_now = std::chrono::steady_clock::now();
+            if (std::chrono::duration<double>(t_now - t_note).count() > 20) {
+                t_note = t_now;
+                progress("scheduler: %d batches after %.1f s", h_ctrl[(g & 1) * C_WORDS + C_CUR],
+                         std::chrono::duration<double>(t_now - t_start).count());
+            }
             for (int r = 0; r < CHUNK; r++, g++) {
                 switch (G) {
                 case 4: k_b_claim<4><<<ggrid, TPB, 0, st>>>(a, g); k_b_viable<4><<<ggrid, TPB, 0, st>>>(a, g, S.viable.p); break;

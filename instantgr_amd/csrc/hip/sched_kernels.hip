@@ -15,6 +15,8 @@
 // oracle's.
 #include "igr_ctx.hpp"
 
+#include <chrono>
+
 #include <algorithm>
 
 namespace igr_dev {
@@ -145,8 +147,16 @@ int run_schedule(igr_ctx* c, const int* h_order, int64_t n, int mode, int* h_bat
     int cur = 0;
     int* h_flag = nullptr;
     HIP_OK(hipHostMalloc(&h_flag, sizeof(int)));
+    const auto t_start = std::chrono::steady_clock::now();
+    auto t_note = t_start;
     try {
         for (;;) {
+            const auto t_now = std::chrono::steady_clock::now();
+            if (std::chrono::duration<double>(t_now - t_note).count() > 20) {
+                t_note = t_now;
+                progress("scheduler: %d batches after %.1f s", cur,
+                         std::chrono::duration<double>(t_now - t_start).count());
+            }
             k_fill<<<blocks(XYc), TPB, 0, st>>>(S.map.p, XYc, -1);
             k_fill<<<blocks(n), TPB, 0, st>>>(S.status.p, n, 0);
             bool have_commit = false;

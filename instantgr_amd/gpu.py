@@ -44,6 +44,29 @@ class RunOpts(C.Structure):
     _fields_ = [("form", C.c_int), ("rrr_iterations", C.c_int), ("threads", C.c_int)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("graph", C.c_int), ("sched_group", C.c_int), ("sched_chunk", C.c_int),
+                ("dag_order", C.c_int), ("dag_waves", C.c_int)]
+
+
+TUNING_FIELDS = [f for f, _ in Tuning._fields_]
+
+
+class ServeOpts(C.Structure):
+    _fields_ = [("threads", C.c_int), ("max_tick_nets", C.c_int64), ("scratch_bytes", C.c_int64),
+                ("hold_ms", C.c_int)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("id", C.c_int64), ("nets", C.c_int64), ("submit_s", C.c_double), ("start_s", C.c_double),
+                ("done_s", C.c_double), ("latency_s", C.c_double), ("tick", C.c_int), ("tick_nets", C.c_int64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("ticks", C.c_int64), ("requests", C.c_int64), ("nets", C.c_int64), ("max_tick_nets", C.c_int64),
+                ("prep_s", C.c_double), ("route_s", C.c_double)]
+
+
 class RunStats(C.Structure):
     _fields_ = [("upload_s", C.c_double), ("pattern_s", C.c_double), ("pattern_schedule_s", C.c_double),
                 ("total_s", C.c_double), ("pattern_batches", C.c_int), ("iterations", C.c_int),
@@ -84,6 +107,14 @@ SIGNATURES = {
     "igr_run": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RunOpts), C.POINTER(RunStats)]),
     "igr_expf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     "igr_copy_probe": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_double)]),
+    "igr_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "igr_server_open": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(ServeOpts), C.POINTER(C.c_void_p)]),
+    "igr_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
+    "igr_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(RequestResult)]),
+    "igr_server_stats_get": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "igr_server_close": (C.c_int, [C.c_void_p]),
+    "igr_server_error": (C.c_char_p, []),
+    "igr_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
 }
 
 
@@ -127,6 +158,22 @@ def copy_probe(nbytes=1 << 30, reps=20):
     g = C.c_double()
     _check(lib().igr_copy_probe(int(nbytes), int(reps), C.byref(g)))
     return g.value
+
+
+def routes_words(flat):
+    """All route words of an (off, data) route set in net order, and the
+    per-net lengths (data[off[i]] words for net i, the length word included)."""
+    off, data = flat
+    lens = data[off[:-1]].astype(np.int64)
+    starts = off[:-1].astype(np.int64)
+    idx = np.repeat(starts - np.concatenate(([0], np.cumsum(lens)[:-1])), lens) + np.arange(lens.sum())
+    return data[idx], lens
+
+
+def routes_equal(a, b):
+    """Route-by-route equality of two (off, data) route sets."""
+    (wa, la), (wb, lb) = routes_words(a), routes_words(b)
+    return bool(np.array_equal(la, lb) and np.array_equal(wa, wb))
 
 
 class Router:
@@ -270,6 +317,23 @@ class Router:
         _check(lib().igr_kernel_bytes(self.h, b))
         return {k: b[i] for i, k in enumerate(KCLASSES)}
 
+    def set_tuning(self, **kw):
+        """Launch-shape overrides of the batched form (igr_tuning fields; 0 = built-in)."""
+        cur = {f: 0 for f in TUNING_FIELDS}
+        cur.update(getattr(self, "_tuning", {}))
+        for k, v in kw.items():
+            if k not in cur:
+                raise KeyError(k)
+            cur[k] = int(v)
+        t = Tuning(*[cur[f] for f in TUNING_FIELDS])
+        _check(lib().igr_set_tuning(self.h, C.byref(t)))
+        self._tuning = cur
+
+    def tuning(self):
+        t = Tuning()
+        _check(lib().igr_get_tuning(self.h, C.byref(t)))
+        return {f: getattr(t, f) for f in TUNING_FIELDS}
+
     def run(self, design, form=FORM_PLAIN, rrr=3, threads=8):
         """Pattern pass plus 'rrr' rip-up-and-reroute passes on a built design."""
         o = RunOpts(form, rrr, threads)
@@ -288,3 +352,50 @@ class Router:
     @property
     def stamp(self):
         return lib().igr_stamp(self.h)
+
+
+def _scheck(rc):
+    if rc != 0:
+        raise RuntimeError("igr server: %s (status %d)" % (lib().igr_server_error().decode(), rc))
+
+
+class Server:
+    """Serving loop over a routed Router (include/igr.h, igr_server_*).
+
+    While open, the worker thread owns the router's context: call nothing else
+    on the router until close()."""
+
+    def __init__(self, router, design, threads=8, max_tick_nets=0, scratch_bytes=0, hold_ms=0):
+        self.router = router
+        self.design = design
+        o = ServeOpts(threads, max_tick_nets, scratch_bytes, hold_ms)
+        h = C.c_void_p()
+        _scheck(lib().igr_server_open(router.h, design.h, C.byref(o), C.byref(h)))
+        self.h = h
+
+    def submit(self, nets):
+        a = np.ascontiguousarray(nets, dtype=np.int32)
+        rid = C.c_int64()
+        _scheck(lib().igr_server_submit(self.h, a.ctypes.data, a.size, C.byref(rid)))
+        return rid.value
+
+    def wait(self, rid):
+        r = RequestResult()
+        _scheck(lib().igr_server_wait(self.h, rid, C.byref(r)))
+        return {f: getattr(r, f) for f, _ in RequestResult._fields_}
+
+    def stats(self):
+        st = ServerStats()
+        _scheck(lib().igr_server_stats_get(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            h, self.h = self.h, None
+            _scheck(lib().igr_server_close(h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
