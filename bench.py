@@ -48,6 +48,11 @@ def parse():
                     help="serving benchmark instead: comma-separated Poisson request rates (req/s) against the "
                          "routed configuration-1 design")
     ap.add_argument("--serve-requests", type=int, default=200, help="requests per rate (serving benchmark)")
+    ap.add_argument("--tuning", default="",
+                    help='launch shapes of the batched form as JSON, e.g. {"graph": 2} (stream launches, as '
+                         'profiler runs use)')
+    ap.add_argument("--cli", action="store_true",
+                    help="also time the command-line stage (files in, route file out) on the same design")
     ap.add_argument("--autotune-nets", type=int, default=0,
                     help="before timing, tune the batched launch shapes on a sample design of this many nets "
                          "from the same generator (untimed)")
@@ -291,7 +296,7 @@ def main():
     t_gen = time.perf_counter()
     design = Design.preset(a.config, nets=a.nets)
     _progress("design generated in %.1f s" % (time.perf_counter() - t_gen))
-    tuning = None
+    tuning = json.loads(a.tuning) if a.tuning else None
     if a.autotune_nets:
         from instantgr_amd import autotune
         t_at = time.perf_counter()
@@ -359,6 +364,26 @@ def main():
             scores[f] = design.evaluate(p)
     if len(forms) == 2:
         same = gpu.routes_equal(routes[forms[0]], routes[forms[1]])
+    cli = None
+    if a.cli:
+        # The drop-in command line: parse .cap/.net, trees, route, write the
+        # route file; wall time from process start to exit, I/O included.
+        import subprocess
+        from instantgr_amd import build as igr_build
+        exe = os.path.join(os.path.dirname(igr_build.__file__), "lib", "igr_route")
+        with tempfile.TemporaryDirectory() as tmp:
+            cap, net, outp = (os.path.join(tmp, n) for n in ("d.cap", "d.net", "d.out"))
+            design.save(cap, net)
+            t_cli = time.perf_counter()
+            proc = subprocess.run([exe, "-cap", cap, "-net", net, "-out", outp, "-form", "batched",
+                                   "-rrr", str(a.rrr), "-threads", str(a.threads)], capture_output=True, text=True)
+            wall = time.perf_counter() - t_cli
+            if proc.returncode != 0:
+                raise RuntimeError("igr_route failed: " + proc.stderr)
+            cli = {"wall_s": wall, "files_mb": (os.path.getsize(cap) + os.path.getsize(net) +
+                                                os.path.getsize(outp)) / 1e6,
+                   "score": design.evaluate(outp)["total"], "stages": proc.stdout.strip()}
+            _progress("command line: %.3f s (%s)" % (wall, cli["stages"]))
     head = "batched" if "batched" in results else forms[-1]
     step_ms = 1e3 * sum(results[head]["step_s"]) / len(results[head]["step_s"])
     value = nets_in / (step_ms * 1e-3)
@@ -375,6 +400,7 @@ def main():
                       "ms_per_step": 1e3 * sum(v["step_s"]) / len(v["step_s"]),
                       "score": scores[f]} for f, v in results.items()},
         "routes_identical": same,
+        "cli": cli,
         "roofline": roofline(results[head]["kernels"], results[head]["bytes"]),
     }
     if copy_gbs is not None:

@@ -209,6 +209,9 @@ typedef struct igr_serve_opts {
     int64_t scratch_bytes;    /* estimated device scratch of the DAG tables per tick (0: no limit) */
     int hold_ms;              /* a tick starts this long after its first request arrived, gathering
                                  later arrivals into it (0: at once) */
+    int overtake;             /* 0: admission stops at the first request that cannot join the tick (the
+                                 ticks keep arrival order); 1: later requests with disjoint footprints
+                                 may pass a waiting one */
 } igr_serve_opts;
 
 typedef struct igr_request_result {

@@ -390,17 +390,18 @@ __device__ void dp_node16(const DagArgs& a, int node, bool valid, int ncw) {
     // per lane in LDS: the child keys (cost above id; padded with INIT_KEY),
     // the best child choices so far (bp) and the current low's (cb), to leave
     // registers for more waves per SIMD
-    __shared__ uint64_t s_key[WTPB][10], s_bp[WTPB][10], s_cb[WTPB][10];
-    uint64_t* key = s_key[lane];
-    uint64_t* bp = s_bp[lane];
-    uint64_t* cb = s_cb[lane];
+    // (child-major, so the 64 lanes of one child entry hit consecutive banks)
+    __shared__ uint64_t s_key[10][WTPB], s_bp[10][WTPB], s_cb[10][WTPB];
+#define key(k) s_key[k][lane]
+#define bp(k) s_bp[k][lane]
+#define cb(k) s_cb[k][lane]
 #pragma unroll
-    for (int k = 0; k < 10; k++) key[k] = (k < nc && act) ? a.packed[(cs + k) * ML + s] : INIT_KEY;
+    for (int k = 0; k < 10; k++) key(k) = (k < nc && act) ? a.packed[(cs + k) * ML + s] : INIT_KEY;
     // lane t of the group holds the via cut cost of layer t at the node
     const float vl = (valid && s < L) ? g.vcost[gidx(g, s, x, y)] : 0.0f;
     double cst = INF;
 #pragma unroll
-    for (int k = 0; k < 10; k++) bp[k] = 0;
+    for (int k = 0; k < 10; k++) bp(k) = 0;
     // Lowest layer: group-uniform bound, the wave runs to its largest.
     int lowmax = fixed_low;
     for (int d = 16; d < 64; d <<= 1) lowmax = max(lowmax, __shfl_xor(lowmax, d, 64));
@@ -415,24 +416,24 @@ __device__ void dp_node16(const DagArgs& a, int node, bool valid, int ncw) {
         double cost = via;
 #pragma unroll
         for (int k = 0; k < 10; k++) {
-            cb[k] = 0;
+            cb(k) = 0;
             if (k >= ncw) continue;
-            const double ck = unpack_cost(key[k]);
+            const double ck = unpack_cost(key(k));
             double m = (run && s >= low && k < nc && ck < INF) ? ck : INF;
-            uint64_t b = (uint64_t)(uint32_t)key[k] * ML + s;
+            uint64_t b = (uint64_t)(uint32_t)key(k) * ML + s;
 #pragma unroll
             for (int d = 1; d < 16; d <<= 1) {
                 const double om = __shfl_up(m, d, 16);
                 const uint64_t ob = __shfl_up(b, d, 16);
                 if (s >= d && !(m < om)) { m = om; b = ob; }
             }
-            cb[k] = b;
+            cb(k) = b;
             if (k < nc) cost += m;
         }
         if (run && s >= low && s >= fixed_high && cost < INF && cost < cst) {
             cst = cost;
 #pragma unroll
-            for (int k = 0; k < 10; k++) bp[k] = cb[k];
+            for (int k = 0; k < 10; k++) bp(k) = cb(k);
         }
         // Suffix argmin over lanes [low, L]: lane s takes the smallest value
         // above it only when strictly smaller.
@@ -448,8 +449,8 @@ __device__ void dp_node16(const DagArgs& a, int node, bool valid, int ncw) {
 #pragma unroll
         for (int k = 0; k < 10; k++) {
             if (k >= ncw) continue;
-            const uint64_t nb = s_bp[(lane & ~15) + src][k];
-            if (run && s >= low && src != s) bp[k] = nb;
+            const uint64_t nb = s_bp[k][(lane & ~15) + src];
+            if (run && s >= low && src != s) bp(k) = nb;
         }
         if (run && s >= low && src != s) cst = nc_cost;
     }
@@ -457,8 +458,11 @@ __device__ void dp_node16(const DagArgs& a, int node, bool valid, int ncw) {
         double* cstp = a.costs + (int64_t)nr * ML;
         cstp[s] = s == 0 ? INF : cst;
         if (s >= 1)
-            for (int k = 0; k < nc; k++) a.best[(cs + k) * ML + s] = bp[k];
+            for (int k = 0; k < nc; k++) a.best[(cs + k) * ML + s] = bp(k);
     }
+#undef key
+#undef bp
+#undef cb
 }
 
 // Candidates of one parent edge from one source layer of the child.

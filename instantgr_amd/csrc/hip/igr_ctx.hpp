@@ -110,6 +110,7 @@ struct Sched {
     // compacted unassigned / active net lists
     DevBuf<int> base, cnt, c_net, ctrl, lists;
     DevBuf<int64_t> head, c_off, bsum;
+    DevBuf<unsigned long long> stat;
 };
 
 // One rip-up-and-reroute pass on the device: DAGs of the rerouted nets in

@@ -57,6 +57,7 @@ struct BArgs {
     int* status;                  // 1 active in this batch, 2 blocked, 0 assigned earlier
     int* ctrl;
     int* lists;                   // NLISTS buffers of n net indices
+    unsigned long long* stat;     // rounds run, active-list entries walked, unassigned at batch starts
     int X, Y, sparsity;
 };
 
@@ -371,6 +372,13 @@ __global__ void __launch_bounds__(TPB) k_b_clear_end(BArgs a, int64_t g, int64_t
         }
     }
     if (t == 0) {
+        if (!done) {
+            int64_t na;
+            (void)act_list(a, st, na);
+            a.stat[0] += 1;
+            a.stat[1] += (unsigned long long)na;
+            if (round == 0) a.stat[2] += (unsigned long long)st[C_NU];
+        }
         int d = st[C_DONE], cur = st[C_CUR], r = st[C_ROUND], have = st[C_HAVE] | (commit ? 1 : 0);
         if (!d) {
             if (!end) {
@@ -454,10 +462,11 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
     S.status.alloc(n);
     S.viable.alloc(n);
     S.lists.alloc(NLISTS * n);
+    S.stat.zero(3, st);
     S.ctrl.zero(2 * C_WORDS + 1, st);
     BArgs a{n, S.ids.p, d_pt_off ? S.pids.p : S.ids.p, d_pt_off ? d_pt_off : N.pt_off.p, d_pt_off ? d_pt : N.pt.p,
             N.hs_off.p, N.hs.p, N.vs_off.p, N.vs.p, S.map.p, S.base.p, S.c_off.p, nullptr, S.head.p,
-            S.batch.p, S.status.p, S.ctrl.p, S.lists.p, c->g.X, c->g.Y,
+            S.batch.p, S.status.p, S.ctrl.p, S.lists.p, S.stat.p, c->g.X, c->g.Y,
             (mode == 1 || mode == 2) ? 2 : 15};
     // Per-cell lists of the nets' point cells, highest priority first.
     k_cell_count<<<blocks(n), TPB, 0, st>>>(a, S.cnt.p);
@@ -528,6 +537,13 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
     }
     const int nbatch = h_ctrl[(g & 1) * C_WORDS + C_CUR];
     HIP_OK(hipHostFree(h_ctrl));
+    {
+        unsigned long long sv[3] = {0, 0, 0};
+        HIP_OK(hipMemcpyAsync(sv, S.stat.p, sizeof(sv), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        progress("scheduler: %d batches, %llu rounds, %.0f active nets per round, %.0f unassigned per batch", nbatch,
+                 sv[0], sv[0] ? (double)sv[1] / (double)sv[0] : 0.0, nbatch ? (double)sv[2] / nbatch : 0.0);
+    }
     std::vector<int> b(n);
     HIP_OK(hipMemcpyAsync(b.data(), S.batch.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));

@@ -6,9 +6,13 @@
 //   2. builds the detour DAGs of the pending requests at the head of the queue
 //      (up to twice the tick budget in nets),
 //   3. admits requests in arrival order while the tick's net and scratch
-//      budgets allow, skipping any request whose footprint meets the
-//      footprint of an earlier request still waiting or admitted (so a
-//      request never overtakes an earlier one it interacts with),
+//      budgets allow and each new request's footprint is disjoint from the
+//      admitted ones'; by default admission stops at the first request that
+//      cannot join, so the ticks route the requests in arrival order and the
+//      result is that of routing them one at a time.  With 'overtake', a
+//      later request may pass a waiting one whose footprint it does not meet
+//      (the waiting request's DAGs are rebuilt next tick and may then reach
+//      further, so this trades the one-at-a-time equivalence for admission),
 //   4. runs one batched rip-up-and-reroute pass over the admitted requests'
 //      subnets together: their nets share the scheduler's batches and the
 //      routing launches.
@@ -180,6 +184,8 @@ void igr_server::tick(std::vector<std::shared_ptr<Request>>& cand) {
             adm.push_back(r);
             nets_in += nn;
             scratch_in += r->scratch;
+        } else if (!o.overtake) {
+            break;   // arrival order: nothing passes a waiting request
         }
         for (int t : r->tiles) claimed[t] = 1;
     }

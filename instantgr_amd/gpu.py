@@ -54,7 +54,7 @@ TUNING_FIELDS = [f for f, _ in Tuning._fields_]
 
 class ServeOpts(C.Structure):
     _fields_ = [("threads", C.c_int), ("max_tick_nets", C.c_int64), ("scratch_bytes", C.c_int64),
-                ("hold_ms", C.c_int)]
+                ("hold_ms", C.c_int), ("overtake", C.c_int)]
 
 
 class RequestResult(C.Structure):
@@ -365,10 +365,10 @@ class Server:
     While open, the worker thread owns the router's context: call nothing else
     on the router until close()."""
 
-    def __init__(self, router, design, threads=8, max_tick_nets=0, scratch_bytes=0, hold_ms=0):
+    def __init__(self, router, design, threads=8, max_tick_nets=0, scratch_bytes=0, hold_ms=0, overtake=False):
         self.router = router
         self.design = design
-        o = ServeOpts(threads, max_tick_nets, scratch_bytes, hold_ms)
+        o = ServeOpts(threads, max_tick_nets, scratch_bytes, hold_ms, 1 if overtake else 0)
         h = C.c_void_p()
         _scheck(lib().igr_server_open(router.h, design.h, C.byref(o), C.byref(h)))
         self.h = h

@@ -60,10 +60,10 @@ def make_requests(design, count, rate, window=50, kmax=5000, seed=0):
     return out
 
 
-def serve(router, design, requests, threads=8, max_tick_nets=20000, scratch_bytes=0, hold_ms=0):
+def serve(router, design, requests, threads=8, max_tick_nets=20000, scratch_bytes=0, hold_ms=0, overtake=False):
     """Submit requests at their arrival offsets; latency and throughput."""
     srv = gpu.Server(router, design, threads=threads, max_tick_nets=max_tick_nets, scratch_bytes=scratch_bytes,
-                     hold_ms=hold_ms)
+                     hold_ms=hold_ms, overtake=overtake)
     ids = []
     t0 = time.perf_counter()
     err = []

@@ -74,8 +74,23 @@ def test_overlapping_requests_follow_arrival_order():
     a = _window_nets(d, 60, 60, 40, 60)
     b = _window_nets(d, 70, 70, 40, 60)       # overlaps a
     c = _window_nets(d, 150, 10, 30, 30)      # apart from both
+    # arrival order (default): b waits for a, and c for b
     r = _routed(d)
     with gpu.Server(r, d, hold_ms=300) as srv:
+        ids = [srv.submit(q) for q in (a, b, c)]
+        res = [srv.wait(i) for i in ids]
+    assert res[0]["tick"] < res[1]["tick"] <= res[2]["tick"]
+    in_order = r.routes_flat()
+    r.close()
+    r2 = _routed(d)
+    with gpu.Server(r2, d) as srv:
+        for q in (a, b, c):
+            srv.wait(srv.submit(q))
+    assert gpu.routes_equal(in_order, r2.routes_flat())
+    r2.close()
+    # overtaking: c, apart from both, joins a's tick; b still follows a
+    r = _routed(d)
+    with gpu.Server(r, d, hold_ms=300, overtake=True) as srv:
         ids = [srv.submit(q) for q in (a, b, c)]
         res = [srv.wait(i) for i in ids]
     assert res[0]["tick"] < res[1]["tick"], "an overlapping later request overtook an earlier one"
