@@ -60,6 +60,9 @@ def parse():
                     help="run a 1 GiB device copy first (bandwidth reference, counter calibration kernel)")
     ap.add_argument("--summarize-stats", metavar="DIR",
                     help="print a summary of the rocprofv3 kernel statistics found under DIR and exit")
+    ap.add_argument("--summarize-lds", metavar="DIR",
+                    help="print the LDS bank-conflict rate per kernel from a rocprofv3 --pmc run of "
+                         "SQ_LDS_BANK_CONFLICT and SQ_LDS_IDX_ACTIVE and exit")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="print the calibrated HBM traffic per dispatch from two rocprofv3 --pmc runs and exit")
     return ap.parse_args()
@@ -236,6 +239,56 @@ def summarize_pmc(fetch_root, write_root, copy_bytes=1 << 30):
     print("\n".join(lines))
 
 
+def summarize_lds(root):
+    """LDS bank-conflict rate per kernel from a rocprofv3 --pmc run of
+    SQ_LDS_BANK_CONFLICT and SQ_LDS_IDX_ACTIVE: conflict cycles over active
+    LDS cycles, summed over the kernel's dispatches."""
+    import csv
+    from collections import defaultdict
+    acc = defaultdict(lambda: [0.0, 0.0, 0])
+    for path in _find(root, "counter_collection.csv"):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                k = _short(r["Kernel_Name"])
+                if r.get("Counter_Name") == "SQ_LDS_BANK_CONFLICT":
+                    acc[k][0] += float(r["Counter_Value"])
+                    acc[k][2] += 1
+                elif r.get("Counter_Name") == "SQ_LDS_IDX_ACTIVE":
+                    acc[k][1] += float(r["Counter_Value"])
+    print("%-24s %10s %18s %18s %12s" % ("kernel", "dispatches", "bank_conflict_cyc", "lds_active_cyc", "conflict"))
+    for k, (c, t, n) in sorted(acc.items(), key=lambda kv: -kv[1][1]):
+        print("%-24s %10d %18.0f %18.0f %12s" % (k, n, c, t, "%.4f" % (c / t) if t > 0 else "no LDS"))
+
+
+def cost_prefix_sweeps(design, reps=20):
+    """Full-grid cost update + prefix sums in both forms, on the benchmark grid
+    and on the largest grid the generator defines (12 metal layers,
+    2500x2500); GB/s over 28 algorithmic bytes per cell."""
+    import numpy as np
+    from instantgr_amd import gpu
+    out = []
+    grids = [("benchmark grid", None), ("12 metal layers, 2500x2500", (11, 2500, 2500))]
+    for name, shape in grids:
+        if shape is None:
+            r = gpu.Router.from_design(design)
+        else:
+            L, X, Y = shape
+            rng = np.random.default_rng(7)
+            cap = rng.integers(2, 7, size=L * X * Y).astype(np.float32)
+            r = gpu.Router(L, X, Y, 1, 1.0, 4.0, np.ones(L), np.ones(X - 1, np.int32), np.ones(Y - 1, np.int32), cap)
+        cells = r.n
+        row = {"grid": name, "cells": cells}
+        for form, key in ((gpu.FORM_PLAIN, "plain"), (gpu.FORM_BATCHED, "fused")):
+            ms = r.cost_sweep_ms(form, reps)
+            gbs = 28.0 * cells / (ms * 1e-3) / 1e9
+            row.update({key + "_us": round(ms * 1e3, 2), key + "_GBs": round(gbs, 1),
+                        key + "_frac": round(gbs / 8000.0, 4)})
+        r.close()
+        out.append(row)
+        _progress("cost + prefix sweep, %s: %s" % (name, json.dumps(row)))
+    return out
+
+
 def serve_bench(a):
     """Serving: the design stays routed on the device; Poisson reroute requests
     of 1-5000 subnets from random 50x50 windows at each rate; p50/p99 latency,
@@ -281,6 +334,9 @@ def main():
         return
     if a.summarize_pmc:
         summarize_pmc(*a.summarize_pmc)
+        return
+    if a.summarize_lds:
+        summarize_lds(a.summarize_lds)
         return
     if a.gpus != 1:
         print("bench.py measures one GPU only: the routing path runs on a single MI355X "
@@ -364,6 +420,7 @@ def main():
             scores[f] = design.evaluate(p)
     if len(forms) == 2:
         same = gpu.routes_equal(routes[forms[0]], routes[forms[1]])
+    sweeps = cost_prefix_sweeps(design)
     cli = None
     if a.cli:
         # The drop-in command line: parse .cap/.net, trees, route, write the
@@ -401,6 +458,7 @@ def main():
                       "score": scores[f]} for f, v in results.items()},
         "routes_identical": same,
         "cli": cli,
+        "cost_prefix_sweep": sweeps,
         "roofline": roofline(results[head]["kernels"], results[head]["bytes"]),
     }
     if copy_gbs is not None:

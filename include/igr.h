@@ -184,6 +184,7 @@ typedef struct igr_tuning {
     int sched_chunk;    /* scheduler rounds enqueued between host checks, 1..256 */
     int dag_order;      /* workgroup order of a DAG batch: 1 deepest DAG first, 2 routing order */
     int dag_waves;      /* DAG kernel register budget: 2 (compiler's choice) or 4 waves per SIMD */
+    int track_k;        /* fused track kernel: cells per thread, 4, 8 or 16 */
 } igr_tuning;
 int igr_set_tuning(igr_ctx* ctx, const igr_tuning* t);
 /* The values in force (built-in choices resolved, per-pass choices as 0). */
@@ -241,6 +242,16 @@ const char* igr_server_error(void);
 
 /* Current commit timestamp counter. */
 int igr_stamp(igr_ctx* ctx);
+
+/* Milliseconds per full-grid cost update plus prefix sums (average of reps
+ * sweeps after a warm-up), on the context's current demand: the plain form's
+ * kernels (update_wcost, update_vcost, compute_presum; src/graph.hpp:40-104,
+ * 405-430) or the batched form's fused track kernel with every track dirty.
+ * The algorithmic bytes of one sweep are 28 per cell (cost update 16: read
+ * capacity and demand, write wire and via cost; prefix sums 12: read wire
+ * cost, write the FP64 sums).  No counterpart in the reference; a
+ * measurement entry point. */
+int igr_cost_sweep(igr_ctx* ctx, int form, int reps, double* ms);
 
 /* Bandwidth reference: a plain 16-byte-vector device copy of 'bytes',
  * repeated 'reps' times; *gbs = (bytes read + written) / time.  Also the

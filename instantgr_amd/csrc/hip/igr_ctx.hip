@@ -153,6 +153,14 @@ int igr_update_cost(igr_ctx* c) {
         HIP_OK(hipStreamSynchronize(c->stream));
     });
 }
+int igr_cost_sweep(igr_ctx* c, int form, int reps, double* ms) {
+    return guard([&]() {
+        if (form != IGR_FORM_PLAIN && form != IGR_FORM_BATCHED) throw std::invalid_argument("unknown form");
+        if (reps < 1 || !ms) throw std::invalid_argument("reps >= 1 and an output");
+        *ms = time_cost_sweep(c, form, reps);
+    });
+}
+
 int igr_compute_presum(igr_ctx* c) {
     return guard([&]() {
         launch_presum(c);
@@ -977,6 +985,10 @@ int igr_set_tuning(igr_ctx* c, const igr_tuning* t) {
         if (t->dag_order < 0 || t->dag_order > 2) throw std::invalid_argument("dag_order: 0, 1 or 2");
         if (t->dag_waves != 0 && t->dag_waves != 2 && t->dag_waves != 4)
             throw std::invalid_argument("dag_waves: 0, 2 or 4");
+        if (t->track_k != 0 && t->track_k != 4 && t->track_k != 8 && t->track_k != 16)
+            throw std::invalid_argument("track_k: 0, 4, 8 or 16");
+        if (t->track_k && t->track_k * 1024 < std::max(c->g.X, c->g.Y))
+            throw std::invalid_argument("track_k too small for this grid's tracks");
         c->tune = *t;
     });
 }
@@ -989,6 +1001,7 @@ int igr_get_tuning(igr_ctx* c, igr_tuning* t) {
         if (!t->sched_chunk) t->sched_chunk = DEF_SCHED_CHUNK;
         if (!t->dag_order) t->dag_order = DEF_DAG_ORDER;
         if (!t->dag_waves) t->dag_waves = DEF_DAG_WAVES;
+        if (!t->track_k) t->track_k = std::max(c->g.X, c->g.Y) <= 4096 ? 4 : 8;
     });
 }
 

@@ -234,7 +234,7 @@ struct igr_ctx {
     igr_dev::Scratch scratch;
     igr_dev::DagDev dag;
     int stamp = 0;
-    igr_tuning tune{0, 0, 0, 0, 0};   // launch-shape overrides (0 = built-in)
+    igr_tuning tune{0, 0, 0, 0, 0, 0};   // launch-shape overrides (0 = built-in)
     double last_dag_prep_s = 0;   // host preparation time of the last DAG pass
     double last_dag_prep_parts[4] = {0, 0, 0, 0};   // fill, batch metadata, upload, allocation
     int64_t cells() const { return (int64_t)g.L * g.X * g.Y; }
@@ -357,5 +357,6 @@ struct BatchedState {
 // Fused per-track pass: commit of the routed batch (s_commit > 0), rip-up of
 // the next one (s_rip > 0), then costs and prefix sums when 'cost' is set.
 void launch_track_fused(igr_ctx* c, int s_commit, int s_rip, bool cost);
+double time_cost_sweep(igr_ctx* c, int form, int reps);
 void launch_rip_scatter(igr_ctx* c, const int* d_nets, int64_t n);
 } // namespace igr_dev

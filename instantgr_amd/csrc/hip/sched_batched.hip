@@ -334,6 +334,7 @@ __global__ void __launch_bounds__(TPB) k_b_clear_end(BArgs a, int64_t g, int64_t
     const int* st = a.ctrl + (g & 1) * C_WORDS;
     int* nx = a.ctrl + ((g + 1) & 1) * C_WORDS;
     const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * TPB;   // grid-stride over cells and list entries
     const bool done = st[C_DONE] != 0;
     const bool commit = st[C_COMMIT] != 0;
     const int round = st[C_ROUND];
@@ -343,30 +344,31 @@ __global__ void __launch_bounds__(TPB) k_b_clear_end(BArgs a, int64_t g, int64_t
     int* nl = a.lists + (int64_t)fresh * a.n;
     if (!done) {
         if (end) {
-            if (t < cells) {
-                int64_t h = a.head[t];
-                const int64_t e = a.c_off[t + 1];
+            for (int64_t q = t; q < cells; q += stride) {
+                int64_t h = a.head[q];
+                const int64_t e = a.c_off[q + 1];
                 while (h < e && a.batch[a.c_net[h]] != -1) h++;
-                a.head[t] = h;
+                a.head[q] = h;
                 const int b = h < e ? a.c_net[h] : -1;
-                a.base[t] = b;
-                a.map[t] = b;
+                a.base[q] = b;
+                a.map[q] = b;
             }
-            if (t < st[C_NU]) {
-                const int i = a.lists[(int64_t)st[C_UL] * a.n + t];
+            const int* ul = a.lists + (int64_t)st[C_UL] * a.n;
+            for (int64_t u = t; u < st[C_NU]; u += stride) {
+                const int i = ul[u];
                 const bool left = a.batch[i] == -1;
                 a.status[i] = left ? 1 : 0;
                 if (left) nl[atomicAdd(&nx[C_NU], 1)] = i;
             }
         } else {
-            if (t < cells) {
-                const int m = a.map[t];
-                if (m >= 0 && a.status[m] == 2) a.map[t] = -1;
+            for (int64_t q = t; q < cells; q += stride) {
+                const int m = a.map[q];
+                if (m >= 0 && a.status[m] == 2) a.map[q] = -1;
             }
             int64_t na;
             const int* al = act_list(a, st, na);
-            if (t < na) {
-                const int i = al[t];
+            for (int64_t u = t; u < na; u += stride) {
+                const int i = al[u];
                 if (active(a, i)) nl[atomicAdd(&nx[C_NA], 1)] = i;
             }
         }
@@ -495,7 +497,7 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
     const int CHUNK = c->tune.sched_chunk ? c->tune.sched_chunk : DEF_SCHED_CHUNK;
     const unsigned ggrid = (unsigned)std::min<int64_t>(2048, blocks(n * G));
     const unsigned wgrid = 256;   // one wave per viable net
-    const unsigned gm = blocks(std::max(XYc, n));
+    const unsigned gm = (unsigned)std::min<int64_t>(2048, blocks(std::max(XYc, n)));   // grid-stride
     int* h_ctrl = nullptr;
     HIP_OK(hipHostMalloc(&h_ctrl, sizeof(int) * (2 * C_WORDS + 1)));
     const int64_t max_rounds = (int64_t)ROUNDS * (n + 1) + CHUNK;

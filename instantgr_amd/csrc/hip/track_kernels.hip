@@ -44,6 +44,63 @@ __device__ __forceinline__ T prev_last(T last, T* sh) {
     return threadIdx.x > 0 ? sh[threadIdx.x - 1] : T(0);
 }
 
+// K consecutive elements of a track per thread.  With K a multiple of 4 and
+// 16-byte aligned tracks (v4), four elements move in one 16-byte access, so a
+// wave reads or writes one contiguous span.
+template <int K, typename T, typename V4>
+__device__ __forceinline__ void load_k(const T* __restrict__ src, int64_t base, int p0, int len, bool v4,
+                                       T (&dst)[K]) {
+    if constexpr (K % 4 == 0) {
+        if (v4) {
+#pragma unroll
+            for (int j = 0; j < K; j += 4) {
+                if (p0 + j < len) {
+                    const V4 q = *reinterpret_cast<const V4*>(src + base + p0 + j);
+                    dst[j] = q.x; dst[j + 1] = q.y; dst[j + 2] = q.z; dst[j + 3] = q.w;
+                } else {
+                    dst[j] = dst[j + 1] = dst[j + 2] = dst[j + 3] = T(0);
+                }
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < K; j++) dst[j] = p0 + j < len ? src[base + p0 + j] : T(0);
+}
+
+template <int K>
+__device__ __forceinline__ void store_k(float* __restrict__ dst, int64_t base, int p0, int len, bool v4,
+                                        const float (&v)[K]) {
+    if constexpr (K % 4 == 0) {
+        if (v4) {
+#pragma unroll
+            for (int j = 0; j < K; j += 4)
+                if (p0 + j < len)
+                    *reinterpret_cast<float4*>(dst + base + p0 + j) = make_float4(v[j], v[j + 1], v[j + 2], v[j + 3]);
+            return;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < K; j++)
+        if (p0 + j < len) dst[base + p0 + j] = v[j];
+}
+
+template <int K>
+__device__ __forceinline__ void store_k(double* __restrict__ dst, int64_t base, int p0, int len, bool v4,
+                                        const double (&v)[K]) {
+    if constexpr (K % 4 == 0) {
+        if (v4) {
+#pragma unroll
+            for (int j = 0; j < K; j += 2)
+                if (p0 + j < len) *reinterpret_cast<double2*>(dst + base + p0 + j) = make_double2(v[j], v[j + 1]);
+            return;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < K; j++)
+        if (p0 + j < len) dst[base + p0 + j] = v[j];
+}
+
 // Via cuts gv at the cells apply sign * (1 or 1/2) to the adjacent edges.
 template <int K>
 __device__ __forceinline__ void apply_vias(float (&dem)[K], const int (&gv)[K], int gnext, int len, float sign) {
@@ -66,8 +123,7 @@ __global__ void __launch_bounds__(B) k_track_fused(GridDev g, int s_c, int s_r, 
                                                    unsigned long long* stats) {
     constexpr int NW = B / 64;
     __shared__ int sh_i[B];
-    __shared__ float sh_f[B];
-    __shared__ double sh_d[B];
+    __shared__ float2 sh_f2[B];
     __shared__ int lds_i[NW];
     __shared__ double lds_d[NW];
     int t = blockIdx.x;
@@ -78,26 +134,24 @@ __global__ void __launch_bounds__(B) k_track_fused(GridDev g, int s_c, int s_r, 
     const int dir = layer_dir(l, g.DIR);
     const int len = dir ? g.Y : g.X;
     const int64_t base = (int64_t)l * g.XYc + (int64_t)t * len;
+    const bool v4 = (len & 3) == 0;
+    const int p0 = threadIdx.x * K;
     float dem[K], cap[K];
     int ts[K];
     uint32_t wr_dem = 0, wr_ts = 0, clr = 0;   // per-element bit masks
+    load_k<K, float, float4>(g.demand, base, p0, len, v4, dem);
+    load_k<K, float, float4>(g.cap, base, p0, len, v4, cap);
+    if (s_c || s_r) load_k<K, int, int4>(g.ts, base, p0, len, v4, ts);
+    else
 #pragma unroll
-    for (int j = 0; j < K; j++) {
-        const int p = threadIdx.x * K + j;
-        const bool in = p < len;
-        dem[j] = in ? g.demand[base + p] : 0.0f;
-        cap[j] = in ? g.cap[base + p] : 0.0f;
-        ts[j] = in ? g.ts[base + p] : 0;
-    }
+        for (int j = 0; j < K; j++) ts[j] = 0;
     if (s_c) {
         int inc[K], gv[K];
+        load_k<K, int, int4>(g.pre, base, p0, len, v4, inc);
+        load_k<K, int, int4>(g.vc, base, p0, len, v4, gv);
 #pragma unroll
-        for (int j = 0; j < K; j++) {
-            const int p = threadIdx.x * K + j;
-            inc[j] = p < len ? g.pre[base + p] : 0;
-            gv[j] = p < len ? g.vc[base + p] : 0;
+        for (int j = 0; j < K; j++)
             if (inc[j] | gv[j]) clr |= 1u << j;
-        }
         block_sklansky<int, K, NW>(inc, lds_i);
 #pragma unroll
         for (int j = 0; j < K; j++) {
@@ -119,15 +173,12 @@ __global__ void __launch_bounds__(B) k_track_fused(GridDev g, int s_c, int s_r, 
     }
     if (s_r) {
         int dec[K], cov[K], gr[K];
+        load_k<K, int, int4>(g.rpre, base, p0, len, v4, dec);
+        load_k<K, int, int4>(g.rcov, base, p0, len, v4, cov);
+        load_k<K, int, int4>(g.rvc, base, p0, len, v4, gr);
 #pragma unroll
-        for (int j = 0; j < K; j++) {
-            const int p = threadIdx.x * K + j;
-            const bool in = p < len;
-            dec[j] = in ? g.rpre[base + p] : 0;
-            cov[j] = in ? g.rcov[base + p] : 0;
-            gr[j] = in ? g.rvc[base + p] : 0;
+        for (int j = 0; j < K; j++)
             if (dec[j] | cov[j] | gr[j]) clr |= 1u << j;
-        }
         block_sklansky<int, K, NW>(dec, lds_i);
         block_sklansky<int, K, NW>(cov, lds_i);
 #pragma unroll
@@ -162,40 +213,55 @@ __global__ void __launch_bounds__(B) k_track_fused(GridDev g, int s_c, int s_r, 
     }
     if (!cost) return;
     // Costs from the new demand (src/graph.hpp:40-71).
+    // Per element the increments of one and of half a track
+    // (inc_cost(.., 1.0) and inc_cost(.., 0.5), sharing the f(d) term); the
+    // via cost takes the element's and its predecessor's, the prefix input is
+    // the predecessor's wire cost.  The predecessor of a thread's first
+    // element is recomputed from the previous thread's last demand and
+    // capacity (one exchange).
     const double w = g.w[l];
     const int* elen = dir ? g.ylen : g.xlen;
-    const float dprev = prev_last(dem[K - 1], sh_f);
-    const float cprev = prev_last(cap[K - 1], sh_f);
-    float wc[K];
-#pragma unroll
-    for (int j = 0; j < K; j++) {
-        const int p = threadIdx.x * K + j;
-        if (p >= len) { wc[j] = 0.0f; continue; }
-        float x = (float)inc_cost(cap[j], dem[j], 1.0, w);
+    auto incs = [&](float c, float d, double& a1, double& ah) {
+        const double f0 = of_cost(c, d);
+        a1 = fmin(1e12, (of_cost(c, (float)((double)d + 1.0 + 0.0)) - f0) * w);
+        ah = fmin(1e12, (of_cost(c, (float)((double)d + 0.5 + 0.0)) - f0) * w);
+    };
+    auto wire = [&](double a1, int p) {
+        float x = (float)a1;
         if (p < len - 1) x = (float)((double)x + g.unit_wire * (double)elen[p]);
-        wc[j] = x;
-        const float dp = j > 0 ? dem[j - 1] : dprev, cp = j > 0 ? cap[j - 1] : cprev;
-        float v;
-        if (p == 0) v = (float)inc_cost(cap[j], dem[j], 1.0, w);
-        else if (p == len - 1) v = (float)inc_cost(cp, dp, 1.0, w);
-        else v = (float)(inc_cost(cap[j], dem[j], 0.5, w) + inc_cost(cp, dp, 0.5, w));
-        g.wcost[base + p] = wc[j];
-        g.vcost[base + p] = (float)((double)v + g.unit_via);
+        return x;
+    };
+    const float2 pdc = prev_last(make_float2(dem[K - 1], cap[K - 1]), sh_f2);
+    double a1p = 0.0, ahp = 0.0;
+    float wcp = 0.0f;
+    if (p0 >= 1 && p0 - 1 < len) {
+        incs(pdc.y, pdc.x, a1p, ahp);
+        wcp = wire(a1p, p0 - 1);
     }
-    // Exclusive FP64 Sklansky prefix of wcost (src/graph.hpp:405-430).
-    const double wprev = prev_last((double)wc[K - 1], sh_d);
+    float wc[K], vc[K];
     double s[K];
 #pragma unroll
-    for (int j = K - 1; j >= 0; j--) {
-        const int p = threadIdx.x * K + j;
-        s[j] = (p >= 1 && p < len) ? (j > 0 ? (double)wc[j - 1] : wprev) : 0.0;
-    }
-    block_sklansky<double, K, NW>(s, lds_d);
-#pragma unroll
     for (int j = 0; j < K; j++) {
-        const int p = threadIdx.x * K + j;
-        if (p < len) g.presum[base + p] = s[j];
+        const int p = p0 + j;
+        if (p >= len) { wc[j] = vc[j] = 0.0f; s[j] = 0.0; continue; }
+        double a1, ah;
+        incs(cap[j], dem[j], a1, ah);
+        wc[j] = wire(a1, p);
+        float v;
+        if (p == 0) v = (float)a1;
+        else if (p == len - 1) v = (float)a1p;
+        else v = (float)(ah + ahp);
+        vc[j] = (float)((double)v + g.unit_via);
+        s[j] = p >= 1 ? (double)wcp : 0.0;   // exclusive prefix input
+        a1p = a1;
+        ahp = ah;
+        wcp = wc[j];
     }
+    store_k<K>(g.wcost, base, p0, len, v4, wc);
+    store_k<K>(g.vcost, base, p0, len, v4, vc);
+    // Exclusive FP64 Sklansky prefix of wcost (src/graph.hpp:405-430).
+    block_sklansky<double, K, NW>(s, lds_d);
+    store_k<K>(g.presum, base, p0, len, v4, s);
     if (threadIdx.x == 0) g.dirty[blockIdx.x] = 0;
 }
 
@@ -277,6 +343,53 @@ void BatchedState::check() {
     if (bad) throw std::runtime_error("a ripped route has a wire across its layer's direction");
 }
 
+// Full-grid sweeps of the cost update and prefix sums, timed with events on
+// the context stream: the plain form's kernels (update_wcost, update_vcost,
+// presum) or the fused track kernel with every track dirty and nothing to
+// commit.  Average milliseconds per sweep over reps sweeps after one warm-up.
+double time_cost_sweep(igr_ctx* c, int form, int reps) {
+    hipStream_t s = c->stream;
+    const int T = c->g.tbase[c->g.L];
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    double ms_total = 0;
+    try {
+        if (form == IGR_FORM_PLAIN) {
+            for (int r = 0; r <= reps; r++) {
+                HIP_OK(hipEventRecord(e0, s));
+                launch_update_cost(c);
+                launch_presum(c);
+                HIP_OK(hipEventRecord(e1, s));
+                HIP_OK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+                if (r > 0) ms_total += ms;
+            }
+        } else {
+            BatchedState bs(c);
+            for (int r = 0; r <= reps; r++) {
+                k_fill_i<<<blocks(T), SB, 0, s>>>(c->dirty.p, T, 1);
+                HIP_OK(hipEventRecord(e0, s));
+                launch_track_fused(c, 0, 0, true);
+                HIP_OK(hipEventRecord(e1, s));
+                HIP_OK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+                if (r > 0) ms_total += ms;
+            }
+            bs.check();
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    HIP_OK(hipEventDestroy(e0));
+    HIP_OK(hipEventDestroy(e1));
+    return ms_total / std::max(1, reps);
+}
+
 void launch_track_fused(igr_ctx* c, int s_commit, int s_rip, bool cost) {
     KScope ks(c, K_TRACK_FUSED);
     const int T = c->g.tbase[c->g.L];
@@ -284,12 +397,26 @@ void launch_track_fused(igr_ctx* c, int s_commit, int s_rip, bool cost) {
     const int cs = cost ? 1 : 0;
     unsigned long long* st = c->track_stats.p;
     hipStream_t s = c->stream;
-    if (ml <= 256) k_track_fused<1, 256><<<T, 256, 0, s>>>(c->g, s_commit, s_rip, cs, st);
-    else if (ml <= 512) k_track_fused<2, 256><<<T, 256, 0, s>>>(c->g, s_commit, s_rip, cs, st);
-    else if (ml <= 1024) k_track_fused<4, 256><<<T, 256, 0, s>>>(c->g, s_commit, s_rip, cs, st);
-    else if (ml <= 2048) k_track_fused<2, 1024><<<T, 1024, 0, s>>>(c->g, s_commit, s_rip, cs, st);
-    else if (ml <= 4096) k_track_fused<4, 1024><<<T, 1024, 0, s>>>(c->g, s_commit, s_rip, cs, st);
-    else k_track_fused<8, 1024><<<T, 1024, 0, s>>>(c->g, s_commit, s_rip, cs, st);
+    // Elements per thread: tuned, or 4 up to 4096-cell tracks and 8 above
+    // (4 measured fastest on 800-, 1500- and 2500-cell tracks); the block is
+    // the smallest of 256, 512, 1024 threads covering a track.
+    const int K = c->tune.track_k ? c->tune.track_k : (ml <= 4096 ? 4 : 8);
+    const int B = K * 256 >= ml ? 256 : K * 512 >= ml ? 512 : 1024;
+    if (K * B < ml) throw std::invalid_argument("track longer than the fused kernel's tile");
+#define IGR_TRACK_LAUNCH(KK, BB) \
+    if (K == KK && B == BB) k_track_fused<KK, BB><<<T, BB, 0, s>>>(c->g, s_commit, s_rip, cs, st)
+    if (!c->tune.track_k && ml <= 256) k_track_fused<1, 256><<<T, 256, 0, s>>>(c->g, s_commit, s_rip, cs, st);
+    else if (!c->tune.track_k && ml <= 512) k_track_fused<2, 256><<<T, 256, 0, s>>>(c->g, s_commit, s_rip, cs, st);
+    else IGR_TRACK_LAUNCH(4, 256);
+    else IGR_TRACK_LAUNCH(4, 512);
+    else IGR_TRACK_LAUNCH(4, 1024);
+    else IGR_TRACK_LAUNCH(8, 256);
+    else IGR_TRACK_LAUNCH(8, 512);
+    else IGR_TRACK_LAUNCH(8, 1024);
+    else IGR_TRACK_LAUNCH(16, 256);
+    else IGR_TRACK_LAUNCH(16, 512);
+    else IGR_TRACK_LAUNCH(16, 1024);
+#undef IGR_TRACK_LAUNCH
     HIP_OK(hipGetLastError());
 }
 

@@ -46,7 +46,7 @@ class RunOpts(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("graph", C.c_int), ("sched_group", C.c_int), ("sched_chunk", C.c_int),
-                ("dag_order", C.c_int), ("dag_waves", C.c_int)]
+                ("dag_order", C.c_int), ("dag_waves", C.c_int), ("track_k", C.c_int)]
 
 
 TUNING_FIELDS = [f for f, _ in Tuning._fields_]
@@ -108,6 +108,7 @@ SIGNATURES = {
     "igr_expf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     "igr_copy_probe": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     "igr_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "igr_cost_sweep": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "igr_server_open": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(ServeOpts), C.POINTER(C.c_void_p)]),
     "igr_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
     "igr_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(RequestResult)]),
@@ -333,6 +334,12 @@ class Router:
         t = Tuning()
         _check(lib().igr_get_tuning(self.h, C.byref(t)))
         return {f: getattr(t, f) for f in TUNING_FIELDS}
+
+    def cost_sweep_ms(self, form=FORM_BATCHED, reps=20):
+        """Milliseconds per full-grid cost update + prefix sums (28 algorithmic bytes per cell)."""
+        ms = C.c_double()
+        _check(lib().igr_cost_sweep(self.h, form, reps, C.byref(ms)))
+        return ms.value
 
     def run(self, design, form=FORM_PLAIN, rrr=3, threads=8):
         """Pattern pass plus 'rrr' rip-up-and-reroute passes on a built design."""
