@@ -185,6 +185,7 @@ typedef struct igr_tuning {
     int dag_order;      /* workgroup order of a DAG batch: 1 deepest DAG first, 2 routing order */
     int dag_waves;      /* DAG kernel register budget: 2 (compiler's choice) or 4 waves per SIMD */
     int track_k;        /* fused track kernel: cells per thread, 4, 8 or 16 */
+    int dag_edges;      /* layer-change costs: 1 a wave per edge with LDS staging, 2 a thread per layer pair */
 } igr_tuning;
 int igr_set_tuning(igr_ctx* ctx, const igr_tuning* t);
 /* The values in force (built-in choices resolved, per-pass choices as 0). */

@@ -9,6 +9,7 @@ results (include/igr.h, igr_tuning):
   dag_order    workgroup order of a DAG batch (deepest DAG first, or routing order)
   dag_waves    register budget of the DAG kernel (2 or 4 waves per SIMD)
   track_k      cells per thread of the fused track kernel (4, 8, 16)
+  dag_edges    layer-change costs: a wave per edge with LDS staging, or a thread per layer pair
 
 `tune` measures them on a design (normally a reduced sample of the production
 design from the same generator) by coordinate descent: each field in turn takes
@@ -32,11 +33,12 @@ CANDIDATES = {
     "dag_order": [1, 2],
     "dag_waves": [2, 4],
     "track_k": [4, 8, 16],
+    "dag_edges": [1, 2],
 }
 
 # Which measured stage each field affects.
 STAGE = {"sched_group": "schedule", "sched_chunk": "schedule", "graph": "route", "dag_order": "route",
-         "dag_waves": "route", "track_k": "route"}
+         "dag_waves": "route", "track_k": "route", "dag_edges": "route"}
 
 
 def _measure(design, tuning, rrr, threads):

@@ -500,6 +500,80 @@ __device__ void dp_edge_layer(const DagArgs& a, int64_t e, int layer) {
 // direction and different from the from layer, from layer along the edge,
 // edges longer than 4 gcells).  Each entry is the same strict first minimum
 // over the same sampled positions as the reference's.
+// Batched form: one wave per DAG edge.  A long straight edge's sample
+// positions (at most 40: dd = dist / min(dist, 20)) are read once into LDS:
+// the via cut costs of every layer at each sample, and the prefix sums of
+// every layer at the samples and at both ends.  The lanes then take the
+// (from, to) layer pairs and form each pair's minimum from LDS with the
+// arithmetic and order of k_dag_edges_pairs: the via cuts summed from the
+// lower layer up, then the two wire pieces, strict '<' keeping the first.
+constexpr int EW_SAMPLES = 40;
+constexpr int EW_WAVES = TPB / 64;
+
+__global__ void __launch_bounds__(TPB) k_dag_edges_wave(DagArgs a, int64_t edges) {
+    __shared__ float s_v[EW_WAVES][EW_SAMPLES][MAXL];      // via cut cost at (layer, sample)
+    __shared__ double s_ps[EW_WAVES][EW_SAMPLES][MAXL];    // prefix sum at (layer, sample)
+    __shared__ double s_pe[EW_WAVES][2][MAXL];             // prefix sum at the from / to end
+    const GridDev& g = a.g;
+    const int L = g.L, LL = L * L;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int64_t er = (int64_t)blockIdx.x * EW_WAVES + wv; er < edges; er += (int64_t)gridDim.x * EW_WAVES) {
+        const int64_t e = a.edge0 + er;
+        const int to = a.par[e], from = a.enode[e];
+        const int tox = a.node[to] / g.Y % g.X, toy = a.node[to] % g.Y;
+        const int fromx = a.node[from] / g.Y % g.X, fromy = a.node[from] % g.Y;
+        const bool vert = fromx == tox;
+        if (!vert && fromy != toy) continue;                   // not straight: no layer change
+        const int distance = abs(tox - fromx) + abs(toy - fromy);
+        if (distance <= 4) continue;
+        const int dd = distance / min(distance, 20);
+        const int s0 = vert ? min(fromy, toy) : min(fromx, tox);
+        const int s1 = vert ? max(fromy, toy) : max(fromx, tox);
+        const int ns = (s1 - 1 - s0) / dd;                     // samples s0 + dd, ..., < s1
+        const int edir = vert ? 1 : 0;                          // direction the wire pieces run
+        // stage: lane = (sample, layer) pairs, then the two ends
+        for (int t = lane; t < ns * L; t += 64) {
+            const int k = t / L, l = t - k * L;
+            const int sp = s0 + (k + 1) * dd;
+            const int sx = vert ? fromx : sp, sy = vert ? sp : fromy;
+            s_v[wv][k][l] = l < L - 1 ? g.vcost[gidx(g, l, sx, sy)] : 0.0f;
+            s_ps[wv][k][l] = layer_dir(l, g.DIR) == edir ? g.presum[gidx(g, l, sx, sy)] : 0.0;
+        }
+        for (int t = lane; t < 2 * L; t += 64) {
+            const int end = t / L, l = t - end * L;
+            const int px = end ? tox : fromx, py = end ? toy : fromy;
+            s_pe[wv][end][l] = layer_dir(l, g.DIR) == edir ? g.presum[gidx(g, l, px, py)] : 0.0;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        const int fpos = vert ? fromy : fromx, tpos = vert ? toy : tox;
+        for (int q = lane; q < LL; q += 64) {
+            const int fl = q / L, tl = q % L;
+            if (tl == fl || (tl % 2) != (fl % 2)) continue;
+            if (layer_dir(fl, g.DIR) != edir) continue;
+            const int lo = min(fl, tl), hi = max(fl, tl);
+            double mn = INF;
+            int decide = -1;
+            for (int k = 0; k < ns; k++) {
+                const int sp = s0 + (k + 1) * dd;
+                double via = 0;
+                for (int ll = lo; ll < hi; ll++) via += (double)s_v[wv][k][ll];
+                double cost = via;
+                // wire piece on fl between the from end and the sample, on tl
+                // between the to end and the sample: presum[max] - presum[min]
+                const double wf = sp >= fpos ? s_ps[wv][k][fl] - s_pe[wv][0][fl] : s_pe[wv][0][fl] - s_ps[wv][k][fl];
+                const double wt = sp >= tpos ? s_ps[wv][k][tl] - s_pe[wv][1][tl] : s_pe[wv][1][tl] - s_ps[wv][k][tl];
+                cost += (wf + wt);
+                if (cost < mn) { mn = cost; decide = sp; }
+            }
+            a.cost_edges[er * LL + q] = mn;
+            a.best_change[er * LL + q] = decide;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 __global__ void __launch_bounds__(TPB) k_dag_edges_pairs(DagArgs a, int64_t edges) {
     const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
     const GridDev& g = a.g;
@@ -658,7 +732,11 @@ void launch_dag_batch_wave(igr_ctx* c, const DagBatch& b, int stamp) {
     if (ne > 0) {
         KScope ke(c, K_DAG_EDGES, b.bytes_edges);
         const int64_t items = ne * c->g.L * c->g.L;
-        k_dag_edges_pairs<<<blocks(items), TPB, 0, c->stream>>>(a, ne);
+        if (c->tune.dag_edges == 2)
+            k_dag_edges_pairs<<<blocks(items), TPB, 0, c->stream>>>(a, ne);
+        else
+            k_dag_edges_wave<<<(unsigned)std::min<int64_t>(8192, (ne + EW_WAVES - 1) / EW_WAVES), TPB, 0,
+                               c->stream>>>(a, ne);
         HIP_OK(hipGetLastError());
     }
     NetLevels lv{D.nl_off.p, D.nlevels.p, D.lvl_node_off.p, D.lvl_edge_off.p, D.dnode.p, D.dedge.p};
