@@ -2,10 +2,14 @@
 //
 // The reference builds trees with FLUTE (src/flute.hpp) whose d>=4 topologies
 // come from POST9.dat, a file absent from the reference tree; this builder
-// therefore constructs its own trees: a rectilinear minimum spanning tree by
-// Prim's algorithm, improved by repeatedly replacing two edges u-v, u-w of a
-// node with the three-point Steiner star at the coordinate-wise median of
-// u, v, w whenever that shortens the tree.  The result is then normalised
+// therefore constructs its own trees.  Nets of two to four pins get an exact
+// rectilinear Steiner minimal tree (the length FLUTE, optimal up to degree 9,
+// also reaches): two and three pins by the median star, four pins by
+// enumerating Hanan-grid Steiner points.  Larger nets get a rectilinear
+// minimum spanning tree by Prim's algorithm, improved by repeatedly replacing
+// two edges u-v, u-w of a node with the three-point Steiner star at the
+// coordinate-wise median of u, v, w whenever that shortens the tree.  The
+// result is then normalised
 // exactly as the reference normalises FLUTE output in my_flute
 // (src/database_cuda.hpp:696-744): node positions sorted and de-duplicated,
 // edges sorted by (length, u, v) and re-spanned with Kruskal; adjacency lists
@@ -165,13 +169,78 @@ void prim(const Coords& C, const std::vector<int>& nx, const std::vector<int>& n
     }
 }
 
+// Length of a minimum spanning tree over at most 6 points (O(n^2) Prim).
+long long small_mst(const Coords& C, const int* x, const int* y, int n) {
+    long long best[6], tot = 0;
+    bool in[6] = {false, false, false, false, false, false};
+    for (int k = 0; k < n; k++) best[k] = C.dist(x[0], y[0], x[k], y[k]);
+    in[0] = true;
+    for (int it = 1; it < n; it++) {
+        int j = -1;
+        for (int k = 0; k < n; k++)
+            if (!in[k] && (j < 0 || best[k] < best[j])) j = k;
+        in[j] = true;
+        tot += best[j];
+        for (int k = 0; k < n; k++)
+            if (!in[k]) best[k] = std::min(best[k], C.dist(x[j], y[j], x[k], y[k]));
+    }
+    return tot;
+}
+
+// Exact rectilinear Steiner minimal tree of four terminals: the spanning tree
+// over the pins plus the best set of at most two Hanan-grid points (Hanan's
+// theorem holds for the L1 metric on the gcell-length coordinates, which are
+// monotone in the gcell indices).  Fewer Steiner points win ties; among equal
+// sets the first in (x, y) order.  Appends the chosen points to nx, ny.
+void rsmt4(const Coords& C, std::vector<int>& nx, std::vector<int>& ny) {
+    int hx[4], hy[4];
+    for (int k = 0; k < 4; k++) { hx[k] = nx[k]; hy[k] = ny[k]; }
+    std::sort(hx, hx + 4);
+    std::sort(hy, hy + 4);
+    const int ux = (int)(std::unique(hx, hx + 4) - hx), uy = (int)(std::unique(hy, hy + 4) - hy);
+    int cx[16], cy[16], m = 0;
+    for (int a = 0; a < ux; a++)
+        for (int b = 0; b < uy; b++) {
+            bool pin = false;
+            for (int k = 0; k < 4; k++) pin = pin || (nx[k] == hx[a] && ny[k] == hy[b]);
+            if (!pin) { cx[m] = hx[a]; cy[m] = hy[b]; m++; }
+        }
+    int x[6] = {nx[0], nx[1], nx[2], nx[3], 0, 0}, y[6] = {ny[0], ny[1], ny[2], ny[3], 0, 0};
+    long long best = small_mst(C, x, y, 4);
+    int s1 = -1, s2 = -1;
+    for (int i = 0; i < m; i++) {
+        x[4] = cx[i]; y[4] = cy[i];
+        const long long l1 = small_mst(C, x, y, 5);
+        if (l1 < best) { best = l1; s1 = i; s2 = -1; }
+    }
+    for (int i = 0; i < m; i++)
+        for (int j = i + 1; j < m; j++) {
+            x[4] = cx[i]; y[4] = cy[i];
+            x[5] = cx[j]; y[5] = cy[j];
+            const long long l2 = small_mst(C, x, y, 6);
+            if (l2 < best) { best = l2; s1 = i; s2 = j; }
+        }
+    if (s1 >= 0) { nx.push_back(cx[s1]); ny.push_back(cy[s1]); }
+    if (s2 >= 0) { nx.push_back(cx[s2]); ny.push_back(cy[s2]); }
+}
+
 // Steiner tree over distinct 2-D positions.
 void steiner_tree(const Coords& C, const std::vector<int>& pts, Tree2D& t) {
     const int n = (int)pts.size();
     std::vector<int> nx(n), ny(n);
     for (int i = 0; i < n; i++) { nx[i] = pts[i] / C.Y; ny[i] = pts[i] % C.Y; }
     std::vector<std::pair<int, int>> edges;
-    if (n >= 2) {
+    if (n == 4) {
+        // exact: spanning tree over the pins and the chosen Steiner points
+        rsmt4(C, nx, ny);
+        std::vector<std::vector<int>> adj(nx.size());
+        const long long span = C.drx.back() + C.dry.back();
+        if (span < INT_MAX / 2) prim<int>(C, nx, ny, adj);
+        else prim<long long>(C, nx, ny, adj);
+        for (int a = 0; a < (int)adj.size(); a++)
+            for (int b : adj[a])
+                if (a < b) edges.emplace_back(a, b);
+    } else if (n >= 2) {
         std::vector<std::vector<int>> adj(n);
         const long long span = C.drx.back() + C.dry.back();
         if (span < INT_MAX / 2) prim<int>(C, nx, ny, adj);

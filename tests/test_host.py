@@ -161,16 +161,19 @@ def test_steiner_trees_connect_and_bounds(small_design):
         if k in (2, 3):
             assert length == hp  # closed form: RSMT of 2 or 3 pins is the half perimeter
             checked[k] += 1
-        elif k in (4, 5) and checked[k] < 60:
+        elif k == 4 and checked[k] < 200:
+            assert length == _rsmt_brute(pxy)  # exact for four pins, as FLUTE's length is
+            checked[k] += 1
+        elif k == 5 and checked[k] < 60:
             opt = _rsmt_brute(pxy)
             assert length >= opt
             ratio.append(length / opt if opt else 1.0)
             checked[k] += 1
-    assert checked[2] > 0 and checked[3] > 0
-    # the tree builder is not FLUTE (optimal up to degree 9): its 4-5 pin
-    # trees stay within 25% of the exact optimum on this design and within 3%
-    # on average; topology parity with FLUTE is unpinned
-    assert max(ratio) <= 1.25 and np.mean(ratio) <= 1.03
+    assert checked[2] > 0 and checked[3] > 0 and checked[4] > 0 and checked[5] > 0
+    # above four pins the tree builder is not FLUTE (optimal up to degree 9):
+    # its 5-pin trees stay within 25% of the exact optimum on this design and
+    # within 4% on average; topology parity with FLUTE is unpinned
+    assert max(ratio) <= 1.25 and np.mean(ratio) <= 1.04
 
 
 def test_subnet_split_covers_pins():
