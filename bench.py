@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # Kernel class -> kernel symbol (the name rocprofv3 reports).
-KERNEL_SYMBOL = {"dag_dp": "k_dag_net", "dag_edges": "k_dag_edges_pairs", "track_fused": "k_track_fused",
+KERNEL_SYMBOL = {"dag_dp": "k_dag_net", "dag_edges": "k_dag_edges_wave", "track_fused": "k_track_fused",
                  "lshape_dp": "k_lshape_plain", "cost_update": "k_update_wcost", "presum": "k_presum",
                  "wire_commit": "k_commit_edges"}
 PROFILES = os.path.join(ROOT, "profiles")

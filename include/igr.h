@@ -186,6 +186,8 @@ typedef struct igr_tuning {
     int dag_waves;      /* DAG kernel register budget: 2 (compiler's choice) or 4 waves per SIMD */
     int track_k;        /* fused track kernel: cells per thread, 4, 8 or 16 */
     int dag_edges;      /* layer-change costs: 1 a wave per edge with LDS staging, 2 a thread per layer pair */
+    int dag_seq;        /* DAG levels with at least this many nodes run a node per lane, narrower ones 16
+                           lanes per node (1 << 30: always 16 lanes) */
 } igr_tuning;
 int igr_set_tuning(igr_ctx* ctx, const igr_tuning* t);
 /* The values in force (built-in choices resolved, per-pass choices as 0). */

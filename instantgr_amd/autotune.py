@@ -10,6 +10,7 @@ results (include/igr.h, igr_tuning):
   dag_waves    register budget of the DAG kernel (2 or 4 waves per SIMD)
   track_k      cells per thread of the fused track kernel (4, 8, 16)
   dag_edges    layer-change costs: a wave per edge with LDS staging, or a thread per layer pair
+  dag_seq      node count from which a DAG level runs a node per lane instead of 16 lanes per node
 
 `tune` measures them on a design (normally a reduced sample of the production
 design from the same generator) by coordinate descent: each field in turn takes
@@ -34,11 +35,12 @@ CANDIDATES = {
     "dag_waves": [2, 4],
     "track_k": [4, 8, 16],
     "dag_edges": [1, 2],
+    "dag_seq": [4, 8, 16, 32, 1 << 30],
 }
 
 # Which measured stage each field affects.
 STAGE = {"sched_group": "schedule", "sched_chunk": "schedule", "graph": "route", "dag_order": "route",
-         "dag_waves": "route", "track_k": "route", "dag_edges": "route"}
+         "dag_waves": "route", "track_k": "route", "dag_edges": "route", "dag_seq": "route"}
 
 
 def _measure(design, tuning, rrr, threads):

@@ -132,13 +132,13 @@ __global__ void __launch_bounds__(TPB) k_dag_edges(DagArgs a, int64_t edges) {
     }
 }
 
-// Bottom-up DP of one node (thread) of the current depth level.
-__global__ void __launch_bounds__(TPB) k_dag_dp(DagArgs a, const int* lvl_nodes, int64_t lo_i, int64_t hi_i) {
-    int64_t t = lo_i + (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (t >= hi_i) return;
+// Bottom-up DP of one node by one thread (Lshape_route_node_layer_change_cuda,
+// src/Lshape_route_detour.hpp:281-353): for each lowest layer, the running
+// minimum of every child slot over the layers above it, the via costs in
+// layer order, then the downward sweep.
+__device__ void dp_node_seq(const DagArgs& a, int node) {
     const GridDev& g = a.g;
-    const int L = g.L, ML = L + 1, LL = L * L, SENT = LL + 1, XY = g.X * g.Y;
-    const int node = lvl_nodes[t];
+    const int L = g.L, ML = L + 1, XY = g.X * g.Y;
     const int nr = (int)(node - a.node0);
     const int x = a.node[node] / g.Y % g.X, y = a.node[node] % g.Y;
     const int fixed_low = 1 + a.node[node] / XY;
@@ -175,6 +175,19 @@ __global__ void __launch_bounds__(TPB) k_dag_dp(DagArgs a, const int* lvl_nodes,
                 for (int k = 0; k < nc; k++) cur_best[k * ML + li] = cur_best[k * ML + li + 1];
             }
     }
+}
+
+// Bottom-up DP of one node (thread) of the current depth level.
+__global__ void __launch_bounds__(TPB) k_dag_dp(DagArgs a, const int* lvl_nodes, int64_t lo_i, int64_t hi_i) {
+    int64_t t = lo_i + (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= hi_i) return;
+    const GridDev& g = a.g;
+    const int L = g.L, ML = L + 1, LL = L * L, SENT = LL + 1, XY = g.X * g.Y;
+    const int node = lvl_nodes[t];
+    const int nr = (int)(node - a.node0);
+    const int x = a.node[node] / g.Y % g.X, y = a.node[node] % g.Y;
+    const double* cst = a.costs + (int64_t)nr * ML;
+    dp_node_seq(a, node);
     const uint32_t myid = (uint32_t)a.loc[node] * ID_MUL;
     for (int64_t e = a.par_off[node]; e < a.par_off[node + 1]; e++) {
         const int p = a.par[e], ci = a.slot[e];
@@ -618,7 +631,7 @@ __global__ void __launch_bounds__(TPB) k_dag_edges_pairs(DagArgs a, int64_t edge
 template <int W>
 __global__ void __launch_bounds__(WTPB) __attribute__((amdgpu_waves_per_eu(W, 8)))
 k_dag_net(DagArgs a, NetLevels lv, const int* nets, const int* list, const int64_t* node_off,
-          const int64_t* route_slot, int* routes, int** rptr) {
+          const int64_t* route_slot, int* routes, int** rptr, int seq_min) {
     const int64_t i = list[blockIdx.x];
     const int lane = threadIdx.x;
     const GridDev& g = a.g;
@@ -637,13 +650,18 @@ k_dag_net(DagArgs a, NetLevels lv, const int* nets, const int* list, const int64
     const int64_t lb = lv.nl_off[i];
     for (int d = nlev - 1; d >= 0; d--) {
         const int64_t a0 = lv.lvl_node_off[lb + d], a1 = lv.lvl_node_off[lb + d + 1];
-        for (int64_t p = a0; p < a1; p += 4) {
-            const int64_t k = p + (lane >> 4);
-            const bool valid = k < a1;
-            const int node = valid ? lv.dnode[k] : 0;
-            int ncw = valid ? a.child_num[node] : 0;
-            for (int o = 16; o < 64; o <<= 1) ncw = max(ncw, __shfl_xor(ncw, o, 64));
-            dp_node16(a, node, valid, ncw);
+        if (a1 - a0 >= seq_min) {
+            // wide level: a node per lane
+            for (int64_t k = a0 + lane; k < a1; k += WTPB) dp_node_seq(a, lv.dnode[k]);
+        } else {
+            for (int64_t p = a0; p < a1; p += 4) {
+                const int64_t k = p + (lane >> 4);
+                const bool valid = k < a1;
+                const int node = valid ? lv.dnode[k] : 0;
+                int ncw = valid ? a.child_num[node] : 0;
+                for (int o = 16; o < 64; o <<= 1) ncw = max(ncw, __shfl_xor(ncw, o, 64));
+                dp_node16(a, node, valid, ncw);
+            }
         }
         __syncthreads();
         const int64_t b0 = lv.lvl_edge_off[lb + d], b1 = lv.lvl_edge_off[lb + d + 1];
@@ -741,14 +759,17 @@ void launch_dag_batch_wave(igr_ctx* c, const DagBatch& b, int stamp) {
     }
     NetLevels lv{D.nl_off.p, D.nlevels.p, D.lvl_node_off.p, D.lvl_edge_off.p, D.dnode.p, D.dedge.p};
     KScope ks(c, K_DAG_DP, b.bytes_dp);
+    // levels with at least seq_min nodes run a node per lane, narrower ones
+    // 16 lanes per node
+    const int seq_min = c->tune.dag_seq ? c->tune.dag_seq : DEF_DAG_SEQ;
     if (c->tune.dag_waves == 4)
         k_dag_net<4><<<(unsigned)n, WTPB, 0, c->stream>>>(a, lv, D.nets.p, D.glist.p + b.net0, D.node_off.p,
                                                           D.route_slot.p, c->nets.dag_routes.back()->p,
-                                                          c->nets.rptr.p);
+                                                          c->nets.rptr.p, seq_min);
     else
         k_dag_net<2><<<(unsigned)n, WTPB, 0, c->stream>>>(a, lv, D.nets.p, D.glist.p + b.net0, D.node_off.p,
                                                           D.route_slot.p, c->nets.dag_routes.back()->p,
-                                                          c->nets.rptr.p);
+                                                          c->nets.rptr.p, seq_min);
     HIP_OK(hipGetLastError());
 }
 

@@ -986,6 +986,7 @@ int igr_set_tuning(igr_ctx* c, const igr_tuning* t) {
         if (t->dag_waves != 0 && t->dag_waves != 2 && t->dag_waves != 4)
             throw std::invalid_argument("dag_waves: 0, 2 or 4");
         if (t->dag_edges < 0 || t->dag_edges > 2) throw std::invalid_argument("dag_edges: 0, 1 or 2");
+        if (t->dag_seq < 0) throw std::invalid_argument("dag_seq: >= 0");
         if (t->track_k != 0 && t->track_k != 4 && t->track_k != 8 && t->track_k != 16)
             throw std::invalid_argument("track_k: 0, 4, 8 or 16");
         if (t->track_k && t->track_k * 1024 < std::max(c->g.X, c->g.Y))
@@ -1003,6 +1004,7 @@ int igr_get_tuning(igr_ctx* c, igr_tuning* t) {
         if (!t->dag_order) t->dag_order = DEF_DAG_ORDER;
         if (!t->dag_waves) t->dag_waves = DEF_DAG_WAVES;
         if (!t->dag_edges) t->dag_edges = 1;
+        if (!t->dag_seq) t->dag_seq = DEF_DAG_SEQ;
         if (!t->track_k) t->track_k = std::max(c->g.X, c->g.Y) <= 4096 ? 4 : 8;
     });
 }

@@ -234,7 +234,7 @@ struct igr_ctx {
     igr_dev::Scratch scratch;
     igr_dev::DagDev dag;
     int stamp = 0;
-    igr_tuning tune{0, 0, 0, 0, 0, 0, 0};   // launch-shape overrides (0 = built-in)
+    igr_tuning tune{0, 0, 0, 0, 0, 0, 0, 0};   // launch-shape overrides (0 = built-in)
     double last_dag_prep_s = 0;   // host preparation time of the last DAG pass
     double last_dag_prep_parts[4] = {0, 0, 0, 0};   // fill, batch metadata, upload, allocation
     int64_t cells() const { return (int64_t)g.L * g.X * g.Y; }
@@ -274,6 +274,7 @@ constexpr int DEF_GRAPH = 1;
 constexpr int DEF_SCHED_CHUNK = 16;
 constexpr int DEF_DAG_ORDER = 1;
 constexpr int DEF_DAG_WAVES = 2;
+constexpr int DEF_DAG_SEQ = 8;
 
 // Records the work enqueued on a stream between construction and launch()
 // as one graph, then launches it; inactive, the work runs as it is enqueued.
