@@ -105,12 +105,18 @@ def _traffic(kernel):
 
 
 def roofline(kt, kb):
-    """Dominant kernel class (by device time in the timed step) and its
-    achieved bandwidth on algorithmic bytes; every class with a byte model is
-    listed beside it."""
-    name, (ms, launches) = max(kt.items(), key=lambda kv: kv[1][0])
+    """Dominant kernel (by device time in the timed step, among the kernel
+    classes with an algorithmic byte model) and its achieved bandwidth on
+    those bytes; every class is listed beside it.  The scheduler class is a
+    sequence of small round kernels timed as one group, with no byte model:
+    when it holds the most time it is named in 'largest_class'."""
+    priced = {k: v for k, v in kt.items() if kb.get(k, 0) > 0 and v[0] > 0}
+    name, (ms, launches) = max((priced or kt).items(), key=lambda kv: kv[1][0])
     out = {"kernel": name, "symbol": KERNEL_SYMBOL.get(name), "total_ms": ms, "launches": launches,
            "bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    top = max(kt.items(), key=lambda kv: kv[1][0])[0]
+    if top != name:
+        out["largest_class"] = top
     b = kb.get(name, 0.0)
     if b > 0 and ms > 0:
         achieved = b / (ms * 1e-3) / 1e9

@@ -188,6 +188,9 @@ typedef struct igr_tuning {
     int dag_edges;      /* layer-change costs: 1 a wave per edge with LDS staging, 2 a thread per layer pair */
     int dag_seq;        /* DAG levels with at least this many nodes run a node per lane, narrower ones 16
                            lanes per node (1 << 30: always 16 lanes) */
+    int sched_kind;     /* scheduler rounds after the first of a batch: 1 active-list (work follows the
+                           active nets), 2 release-driven (work follows the released cells); built-in
+                           choice 2 from 1.5M scheduled nets */
 } igr_tuning;
 int igr_set_tuning(igr_ctx* ctx, const igr_tuning* t);
 /* The values in force (built-in choices resolved, per-pass choices as 0). */

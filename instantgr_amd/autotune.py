@@ -11,6 +11,8 @@ results (include/igr.h, igr_tuning):
   track_k      cells per thread of the fused track kernel (4, 8, 16)
   dag_edges    layer-change costs: a wave per edge with LDS staging, or a thread per layer pair
   dag_seq      node count from which a DAG level runs a node per lane instead of 16 lanes per node
+  sched_kind   scheduler rounds: active-list (work follows the active nets) or release-driven
+               (work follows the released cells)
 
 `tune` measures them on a design (normally a reduced sample of the production
 design from the same generator) by coordinate descent: each field in turn takes
@@ -36,11 +38,13 @@ CANDIDATES = {
     "track_k": [4, 8, 16],
     "dag_edges": [1, 2],
     "dag_seq": [4, 8, 16, 32, 1 << 30],
+    "sched_kind": [1, 2],
 }
 
 # Which measured stage each field affects.
 STAGE = {"sched_group": "schedule", "sched_chunk": "schedule", "graph": "route", "dag_order": "route",
-         "dag_waves": "route", "track_k": "route", "dag_edges": "route", "dag_seq": "route"}
+         "dag_waves": "route", "track_k": "route", "dag_edges": "route", "dag_seq": "route",
+         "sched_kind": "schedule"}
 
 
 def _measure(design, tuning, rrr, threads):

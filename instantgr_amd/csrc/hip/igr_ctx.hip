@@ -987,6 +987,7 @@ int igr_set_tuning(igr_ctx* c, const igr_tuning* t) {
             throw std::invalid_argument("dag_waves: 0, 2 or 4");
         if (t->dag_edges < 0 || t->dag_edges > 2) throw std::invalid_argument("dag_edges: 0, 1 or 2");
         if (t->dag_seq < 0) throw std::invalid_argument("dag_seq: >= 0");
+        if (t->sched_kind < 0 || t->sched_kind > 2) throw std::invalid_argument("sched_kind: 0, 1 or 2");
         if (t->track_k != 0 && t->track_k != 4 && t->track_k != 8 && t->track_k != 16)
             throw std::invalid_argument("track_k: 0, 4, 8 or 16");
         if (t->track_k && t->track_k * 1024 < std::max(c->g.X, c->g.Y))

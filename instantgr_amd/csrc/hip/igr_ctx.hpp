@@ -107,8 +107,9 @@ struct NetsDev {
 struct Sched {
     DevBuf<int> map, batch, status, viable, ids, pids, flags;
     // batched form: per-cell net lists, their heads and the round-0 map; the
-    // compacted unassigned / active net lists
-    DevBuf<int> base, cnt, c_net, ctrl, lists;
+    // compacted unassigned / active net lists; release-driven rounds: the
+    // released cells, new owners, committed nets and their cells, with stamps
+    DevBuf<int> base, cnt, c_net, ctrl, lists, rel, cand, cstamp, comm, hits, hstamp;
     DevBuf<int64_t> head, c_off, bsum;
     DevBuf<unsigned long long> stat;
 };
@@ -274,7 +275,10 @@ constexpr int DEF_GRAPH = 1;
 constexpr int DEF_SCHED_CHUNK = 16;
 constexpr int DEF_DAG_ORDER = 1;
 constexpr int DEF_DAG_WAVES = 2;
-constexpr int DEF_DAG_SEQ = 8;
+constexpr int DEF_DAG_SEQ = 16;
+// Batched scheduler: release-driven rounds from this many scheduled nets,
+// active-list rounds below.
+constexpr int64_t DEF_SCHED_RELEASE_NETS = 1500000;
 
 // Records the work enqueued on a stream between construction and launch()
 // as one graph, then launches it; inactive, the work runs as it is enqueued.

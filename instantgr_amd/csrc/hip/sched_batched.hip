@@ -9,14 +9,30 @@
 //    sorted by priority (built once per pass) and a head index per cell that
 //    moves past assigned nets at the end of each batch, so round 0 claims
 //    nothing.
-//  * In later rounds a point claim can only change a cell the previous
-//    round's clear emptied (values otherwise only grow within a batch), so a
-//    net claims a point only where the map holds a lower value.
-//  * The round kernels walk compacted net lists instead of every net: the
-//    unassigned nets (rebuilt at each batch end) and the nets still active in
-//    the batch (rebuilt after each round that committed, dropping committed
-//    and blocked nets).  The list order is free: every decision is an atomic
-//    maximum or a test against the map.
+//  * Within a batch a cell's value only grows (claims are maxima) except where
+//    the end of a round releases the cells of blocked nets, so a later point
+//    claim can only change a released cell.
+//  * Two ways to run the later rounds give the same batches at different
+//    costs per round:
+//    - active-list rounds (kernels k_a_*): the nets still active in the batch
+//      form a compacted list rebuilt after each round that committed; they
+//      claim their points where the map holds a lower value, and the revert
+//      tests each of them against the owners of its points.  Work per round
+//      follows the active nets' point counts.
+//    - release-driven rounds (kernels k_r_*): the end of a round lists the
+//      cells it released; each goes to its highest-priority active lister (the
+//      first active entry of the cell's list, scanned by a lane group), and
+//      only those new owners are tested for viability (a net can only turn
+//      viable by taking a released cell).  A net is blocked when one of its
+//      points is owned by a net committed in this batch; owners committed in
+//      earlier rounds already blocked every active net they could (their
+//      cells only shrink), so the revert lists the cells of the nets committed
+//      in this round and blocks the active nets in those cells' lists.  Work
+//      per round follows the released cells and committed nets.
+//    On a 640k-cell grid with 0.5M nets the active-list rounds cost less; on a
+//    2.25M-cell grid with 3M-10M nets the release-driven ones do.
+//  * List orders are free: every decision is an atomic maximum or a test
+//    against the map.
 //  * The round and batch control lives in a device word block; the host
 //    enqueues rounds in chunks and reads back one flag per chunk.  Kernels of
 //    rounds after the end return at once.
@@ -33,10 +49,14 @@ constexpr int TPB = 256;
 constexpr int ROUNDS = 30;        // rounds per batch (src/database_cuda.hpp:612)
 
 // control state words (two slots of C_WORDS, then the placed-net counter).
-// C_NA / C_AL: length and buffer of the batch's active list (C_NA < 0: the
-// unassigned list itself); C_NU / C_UL: the same for the unassigned list.
-enum { C_DONE = 0, C_CUR, C_ROUND, C_COMMIT, C_HAVE, C_NVIABLE, C_NA, C_AL, C_NU, C_UL, C_WORDS };
-constexpr int NLISTS = 4;         // list buffers of n entries each
+// C_NU / C_UL: length and buffer of the unassigned list.  Active-list rounds:
+// C_NA / C_AL, the same for the batch's active list (C_NA < 0: the unassigned
+// list itself).  Release-driven rounds: C_NR, cells the previous round
+// released (list rel); C_NK, new owners of those cells (list cand); C_NC, nets
+// committed in this round (list comm); C_NH, cells those nets own (list hits).
+enum { C_DONE = 0, C_CUR, C_ROUND, C_COMMIT, C_HAVE, C_NVIABLE, C_NU, C_UL, C_NA, C_AL, C_NR, C_NK, C_NC, C_NH,
+       C_WORDS };
+constexpr int NLISTS = 3;         // net list buffers of n entries each
 
 struct BArgs {
     int64_t n;
@@ -57,14 +77,32 @@ struct BArgs {
     int* status;                  // 1 active in this batch, 2 blocked, 0 assigned earlier
     int* ctrl;
     int* lists;                   // NLISTS buffers of n net indices
-    unsigned long long* stat;     // rounds run, active-list entries walked, unassigned at batch starts
+    // release-driven rounds only
+    int* rel;                     // X*Y cells released by the previous round
+    int* cand;                    // nets that took a released cell this round
+    int* cstamp;                  // per net: 1 + the round that listed it in cand
+    int* comm;                    // nets committed this round
+    int* hits;                    // X*Y cells owned by the nets committed this round
+    int* hstamp;                  // per cell: 1 + the round that listed it in hits
+    unsigned long long* stat;     // rounds run, nets tested for viability, unassigned at batch starts
     int X, Y, sparsity;
 };
 
-// The batch's active list of the round whose state is st.
-__device__ __forceinline__ const int* act_list(const BArgs& a, const int* st, int64_t& na) {
+// Active-list rounds: the batch's active list of the round whose state is st.
+__device__ __forceinline__ const int* a_list(const BArgs& a, const int* st, int64_t& na) {
     na = st[C_NA] < 0 ? st[C_NU] : st[C_NA];
     return a.lists + (int64_t)st[C_AL] * a.n;
+}
+
+// Release-driven rounds: the nets whose viability the round whose state is st
+// tests: every unassigned net in round 0, the new owners of released cells later.
+__device__ __forceinline__ const int* r_list(const BArgs& a, const int* st, int64_t& na) {
+    if (st[C_ROUND] == 0) {
+        na = st[C_NU];
+        return a.lists + (int64_t)st[C_UL] * a.n;
+    }
+    na = st[C_NK];
+    return a.cand;
 }
 
 __device__ __forceinline__ bool active(const BArgs& a, int64_t i) { return a.status[i] == 1 && a.batch[i] == -1; }
@@ -195,60 +233,21 @@ __device__ __forceinline__ uint64_t group_bits(uint64_t m, int G) {
     return (m >> base) & mask;
 }
 
-// 1. point claims of the active nets (from round 1; round 0 starts from the
-//    kept map): only cells holding a lower value can change
-template <int G>
-__global__ void __launch_bounds__(TPB) k_b_claim(BArgs a, int64_t g) {
-    const int* st = a.ctrl + (g & 1) * C_WORDS;
-    if (st[C_DONE] || st[C_ROUND] == 0) return;
-    int64_t na;
-    const int* al = act_list(a, st, na);
-    const int64_t nth = (int64_t)gridDim.x * TPB / G;
-    const int lg = threadIdx.x & (G - 1);
-    for (int64_t t = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G; t < na; t += nth) {
-        const int64_t i = al[t];
-        if (!active(a, i)) continue;
-        const int64_t k1 = a.pt_off[a.pids[i] + 1];
-        for (int64_t k = a.pt_off[a.pids[i]] + lg; k < k1; k += G) {
-            int* m = &a.map[a.pt[k]];
-            if (*m < (int)i) atomicMax(m, (int)i);
-        }
-    }
+// Appends v to list for every lane with want, one counter atomic per wave.
+// Every lane of the wave calls it.
+__device__ __forceinline__ void wave_append(bool want, int v, int* counter, int* list) {
+    const uint64_t m = __ballot(want);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __builtin_ctzll(m);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader);
+    if (want) list[base + __popcll(m & ((1ull << lane) - 1))] = v;
 }
 
-// 2. active nets owning every point cell go to the viable list
-template <int G>
-__global__ void __launch_bounds__(TPB) k_b_viable(BArgs a, int64_t g, int* vlist) {
-    int* st = a.ctrl + (g & 1) * C_WORDS;
-    if (st[C_DONE]) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        // the successor lists' counters, filled by this round's last kernel
-        int* nx = a.ctrl + ((g + 1) & 1) * C_WORDS;
-        nx[C_NA] = 0;
-        nx[C_NU] = 0;
-    }
-    int64_t na;
-    const int* al = act_list(a, st, na);
-    const int64_t nth = (int64_t)gridDim.x * TPB / G;
-    const int lg = threadIdx.x & (G - 1);
-    const int64_t rounds = (na + nth - 1) / nth;   // the same for every lane of the wave
-    for (int64_t r = 0; r < rounds; r++) {
-        const int64_t t = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G + r * nth;
-        const int64_t i = t < na ? al[t] : 0;
-        const bool live = t < na && active(a, i);
-        const int64_t k0 = live ? a.pt_off[a.pids[i]] : 0, k1 = live ? a.pt_off[a.pids[i] + 1] : 0;
-        bool fail = !live;
-        for (int64_t k = k0 + lg;; k += G) {
-            const bool f = !fail && k < k1 && a.map[a.pt[k]] != (int)i;
-            fail = fail || group_bits(__ballot(f), G) != 0;
-            if (__ballot(!fail && k + G < k1) == 0) break;
-        }
-        if (lg == 0 && !fail) vlist[atomicAdd(&st[C_NVIABLE], 1)] = (int)i;
-    }
-}
-
-// 3. a wave per viable net: neighbour cells (0,-1), (-1,0), (1,0) of its
-//    points and every sparsity-th cell of its segments
+// 3. (both) a wave per viable net claims the neighbour cells (0,-1), (-1,0),
+//    (1,0) of its points and every sparsity-th cell of its segments
 __global__ void __launch_bounds__(TPB) k_b_claim_rest(BArgs a, int64_t g, const int* vlist) {
     const int* st = a.ctrl + (g & 1) * C_WORDS;
     if (st[C_DONE]) return;
@@ -275,8 +274,61 @@ __global__ void __launch_bounds__(TPB) k_b_claim_rest(BArgs a, int64_t g, const 
     }
 }
 
+// ---- active-list rounds -----------------------------------------------------
+// 1. point claims of the active nets (from round 1; round 0 starts from the
+//    kept map): only cells holding a lower value can change
+template <int G>
+__global__ void __launch_bounds__(TPB) k_a_claim(BArgs a, int64_t g) {
+    const int* st = a.ctrl + (g & 1) * C_WORDS;
+    if (st[C_DONE] || st[C_ROUND] == 0) return;
+    int64_t na;
+    const int* al = a_list(a, st, na);
+    const int64_t nth = (int64_t)gridDim.x * TPB / G;
+    const int lg = threadIdx.x & (G - 1);
+    for (int64_t t = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G; t < na; t += nth) {
+        const int64_t i = al[t];
+        if (!active(a, i)) continue;
+        const int64_t k1 = a.pt_off[a.pids[i] + 1];
+        for (int64_t k = a.pt_off[a.pids[i]] + lg; k < k1; k += G) {
+            int* m = &a.map[a.pt[k]];
+            if (*m < (int)i) atomicMax(m, (int)i);
+        }
+    }
+}
+
+// 2. active nets owning every point cell go to the viable list
+template <int G>
+__global__ void __launch_bounds__(TPB) k_a_viable(BArgs a, int64_t g, int* vlist) {
+    int* st = a.ctrl + (g & 1) * C_WORDS;
+    if (st[C_DONE]) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the successor lists' counters, filled by this round's last kernel
+        int* nx = a.ctrl + ((g + 1) & 1) * C_WORDS;
+        nx[C_NA] = 0;
+        nx[C_NU] = 0;
+    }
+    int64_t na;
+    const int* al = a_list(a, st, na);
+    const int64_t nth = (int64_t)gridDim.x * TPB / G;
+    const int lg = threadIdx.x & (G - 1);
+    const int64_t rounds = (na + nth - 1) / nth;   // the same for every lane of the wave
+    for (int64_t r = 0; r < rounds; r++) {
+        const int64_t t = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G + r * nth;
+        const int64_t i = t < na ? al[t] : 0;
+        const bool live = t < na && active(a, i);
+        const int64_t k0 = live ? a.pt_off[a.pids[i]] : 0, k1 = live ? a.pt_off[a.pids[i] + 1] : 0;
+        bool fail = !live;
+        for (int64_t k = k0 + lg;; k += G) {
+            const bool f = !fail && k < k1 && a.map[a.pt[k]] != (int)i;
+            fail = fail || group_bits(__ballot(f), G) != 0;
+            if (__ballot(!fail && k + G < k1) == 0) break;
+        }
+        if (lg == 0 && !fail) vlist[atomicAdd(&st[C_NVIABLE], 1)] = (int)i;
+    }
+}
+
 // 4. a wave per viable net: commit it if it still owns every point cell
-__global__ void __launch_bounds__(TPB) k_b_commit(BArgs a, int64_t g, const int* vlist) {
+__global__ void __launch_bounds__(TPB) k_a_commit(BArgs a, int64_t g, const int* vlist) {
     int* st = a.ctrl + (g & 1) * C_WORDS;
     if (st[C_DONE]) return;
     const int nv = st[C_NVIABLE];
@@ -297,11 +349,11 @@ __global__ void __launch_bounds__(TPB) k_b_commit(BArgs a, int64_t g, const int*
 // 5. after a commit: active nets with a point cell owned by a committed net
 //    sit out the batch
 template <int G>
-__global__ void __launch_bounds__(TPB) k_b_revert(BArgs a, int64_t g) {
+__global__ void __launch_bounds__(TPB) k_a_revert(BArgs a, int64_t g) {
     const int* st = a.ctrl + (g & 1) * C_WORDS;
     if (st[C_DONE] || !st[C_COMMIT]) return;
     int64_t na;
-    const int* al = act_list(a, st, na);
+    const int* al = a_list(a, st, na);
     const int64_t nth = (int64_t)gridDim.x * TPB / G;
     const int lg = threadIdx.x & (G - 1);
     const int64_t rounds = (na + nth - 1) / nth;
@@ -330,7 +382,7 @@ __global__ void __launch_bounds__(TPB) k_b_revert(BArgs a, int64_t g) {
 //    the kept map with every unassigned net active.  The successor lists go
 //    to a buffer neither current list uses.  Thread 0 writes the successor
 //    control state.
-__global__ void __launch_bounds__(TPB) k_b_clear_end(BArgs a, int64_t g, int64_t cells) {
+__global__ void __launch_bounds__(TPB) k_a_clear_end(BArgs a, int64_t g, int64_t cells) {
     const int* st = a.ctrl + (g & 1) * C_WORDS;
     int* nx = a.ctrl + ((g + 1) & 1) * C_WORDS;
     const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
@@ -366,7 +418,7 @@ __global__ void __launch_bounds__(TPB) k_b_clear_end(BArgs a, int64_t g, int64_t
                 if (m >= 0 && a.status[m] == 2) a.map[q] = -1;
             }
             int64_t na;
-            const int* al = act_list(a, st, na);
+            const int* al = a_list(a, st, na);
             for (int64_t u = t; u < na; u += stride) {
                 const int i = al[u];
                 if (active(a, i)) nl[atomicAdd(&nx[C_NA], 1)] = i;
@@ -376,7 +428,7 @@ __global__ void __launch_bounds__(TPB) k_b_clear_end(BArgs a, int64_t g, int64_t
     if (t == 0) {
         if (!done) {
             int64_t na;
-            (void)act_list(a, st, na);
+            (void)a_list(a, st, na);
             a.stat[0] += 1;
             a.stat[1] += (unsigned long long)na;
             if (round == 0) a.stat[2] += (unsigned long long)st[C_NU];
@@ -417,6 +469,256 @@ __global__ void __launch_bounds__(TPB) k_b_clear_end(BArgs a, int64_t g, int64_t
     }
 }
 
+// ---- release-driven rounds --------------------------------------------------
+// 1. from round 1: each cell the previous round released goes to its
+//    highest-priority active lister (the first active entry of its list),
+//    which joins the viability test.  A group of G lanes scans a cell's list
+//    G entries a step.
+template <int G>
+__global__ void __launch_bounds__(TPB) k_r_claim(BArgs a, int64_t g) {
+    int* st = a.ctrl + (g & 1) * C_WORDS;
+    if (st[C_DONE] || st[C_ROUND] == 0) return;
+    const int64_t nr = st[C_NR];
+    const int64_t nth = (int64_t)gridDim.x * TPB / G;
+    const int lg = threadIdx.x & (G - 1);
+    const int64_t rounds = (nr + nth - 1) / nth;   // the same for every lane of the wave
+    for (int64_t r = 0; r < rounds; r++) {
+        const int64_t t = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G + r * nth;
+        const bool live = t < nr;
+        const int q = live ? a.rel[t] : 0;
+        int64_t h = live ? a.head[q] : 0;
+        const int64_t e = live ? a.c_off[q + 1] : 0;
+        bool found = !live;
+        for (;;) {
+            const int64_t k = h + lg;
+            int j = -1;
+            bool act = false;
+            if (!found && k < e) {
+                j = a.c_net[k];
+                act = active(a, j);
+            }
+            const uint64_t m = group_bits(__ballot(act), G);
+            const bool mine = m != 0 && lg == __builtin_ctzll(m);
+            bool fresh = false;
+            if (mine) {
+                a.map[q] = j;
+                fresh = atomicExch(&a.cstamp[j], (int)g + 1) != (int)g + 1;
+            }
+            wave_append(fresh, j, &st[C_NK], a.cand);
+            found = found || m != 0;
+            h += G;
+            if (__ballot(!found && h < e) == 0) break;
+        }
+    }
+}
+
+// 2. active nets owning every point cell go to the viable list
+template <int G>
+__global__ void __launch_bounds__(TPB) k_r_viable(BArgs a, int64_t g, int* vlist) {
+    int* st = a.ctrl + (g & 1) * C_WORDS;
+    if (st[C_DONE]) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the successor lists' counters, filled by this round's last kernel
+        int* nx = a.ctrl + ((g + 1) & 1) * C_WORDS;
+        nx[C_NU] = 0;
+        nx[C_NR] = 0;
+    }
+    int64_t na;
+    const int* al = r_list(a, st, na);
+    const int64_t nth = (int64_t)gridDim.x * TPB / G;
+    const int lg = threadIdx.x & (G - 1);
+    const int64_t rounds = (na + nth - 1) / nth;   // the same for every lane of the wave
+    for (int64_t r = 0; r < rounds; r++) {
+        const int64_t t = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G + r * nth;
+        const int64_t i = t < na ? al[t] : 0;
+        const bool live = t < na && active(a, i);
+        const int64_t k0 = live ? a.pt_off[a.pids[i]] : 0, k1 = live ? a.pt_off[a.pids[i] + 1] : 0;
+        bool fail = !live;
+        for (int64_t k = k0 + lg;; k += G) {
+            const bool f = !fail && k < k1 && a.map[a.pt[k]] != (int)i;
+            fail = fail || group_bits(__ballot(f), G) != 0;
+            if (__ballot(!fail && k + G < k1) == 0) break;
+        }
+        if (lg == 0 && !fail) vlist[atomicAdd(&st[C_NVIABLE], 1)] = (int)i;
+    }
+}
+
+// 4. a wave per viable net: commit it if it still owns every point cell
+__global__ void __launch_bounds__(TPB) k_r_commit(BArgs a, int64_t g, const int* vlist) {
+    int* st = a.ctrl + (g & 1) * C_WORDS;
+    if (st[C_DONE]) return;
+    const int nv = st[C_NVIABLE];
+    const int lane = threadIdx.x & 63;
+    for (int w = (blockIdx.x * TPB + threadIdx.x) >> 6; w < nv; w += (gridDim.x * TPB) >> 6) {
+        const int i = vlist[w];
+        bool lost = false;
+        for (int64_t k = a.pt_off[a.pids[i]] + lane; k < a.pt_off[a.pids[i] + 1]; k += 64)
+            lost = lost || a.map[a.pt[k]] != i;
+        if (__ballot(lost) == 0 && lane == 0) {
+            a.comm[atomicAdd(&st[C_NC], 1)] = i;
+            a.batch[i] = st[C_CUR];
+            atomicAdd(&a.ctrl[2 * C_WORDS], 1);   // nets placed so far
+            st[C_COMMIT] = 1;
+        }
+    }
+}
+
+// 5a. after a commit: the cells the nets committed in this round own.  A
+//     wave per committed net walks the cells it may own (points, their
+//     neighbours, sampled segment cells) and lists those it still owns, once
+//     each.
+__global__ void __launch_bounds__(TPB) k_r_hits(BArgs a, int64_t g) {
+    int* st = a.ctrl + (g & 1) * C_WORDS;
+    if (st[C_DONE] || !st[C_COMMIT]) return;
+    const int lane = threadIdx.x & 63;
+    const int X = a.X, Y = a.Y;
+    const int stamp = (int)g + 1;
+    auto hit = [&](int q, int c) {
+        if (a.map[q] == c && atomicExch(&a.hstamp[q], stamp) != stamp) a.hits[atomicAdd(&st[C_NH], 1)] = q;
+    };
+    for (int w = (blockIdx.x * TPB + threadIdx.x) >> 6; w < st[C_NC]; w += (gridDim.x * TPB) >> 6) {
+        const int c = a.comm[w];
+        const int net = a.ids[c];
+        for (int64_t k = a.pt_off[a.pids[c]] + lane; k < a.pt_off[a.pids[c] + 1]; k += 64) {
+            const int p = a.pt[k], x = p / Y, y = p % Y;
+            hit(p, c);
+            if (y - 1 >= 0) hit(x * Y + y - 1, c);
+            if (x - 1 >= 0) hit((x - 1) * Y + y, c);
+            if (x + 1 < X) hit((x + 1) * Y + y, c);
+        }
+        for (int64_t k = a.hs_off[net] + lane; k < a.hs_off[net + 1]; k += 64) {
+            const int s0 = a.hs[2 * k], e = a.hs[2 * k + 1], y = s0 % Y;
+            for (int x = s0 / Y; x <= e / Y; x += a.sparsity) hit(x * Y + y, c);
+        }
+        for (int64_t k = a.vs_off[net] + lane; k < a.vs_off[net + 1]; k += 64) {
+            const int s0 = a.vs[2 * k], e = a.vs[2 * k + 1], x = s0 / Y;
+            for (int y = s0 % Y; y <= e % Y; y += a.sparsity) hit(x * Y + y, c);
+        }
+    }
+}
+
+// 5b. every active net listing one of those cells sits out the batch (a net
+//     is blocked when one of its point cells is owned by a net committed in
+//     this batch; owners committed in earlier rounds blocked their listers
+//     then, and their cells only shrink).  A group of G lanes per cell.
+template <int G>
+__global__ void __launch_bounds__(TPB) k_r_block(BArgs a, int64_t g) {
+    const int* st = a.ctrl + (g & 1) * C_WORDS;
+    if (st[C_DONE] || !st[C_COMMIT]) return;
+    const int64_t nh = st[C_NH];
+    const int64_t nth = (int64_t)gridDim.x * TPB / G;
+    const int lg = threadIdx.x & (G - 1);
+    for (int64_t t = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G; t < nh; t += nth) {
+        const int q = a.hits[t];
+        const int64_t e = a.c_off[q + 1];
+        for (int64_t k = a.head[q] + lg; k < e; k += G) {
+            const int j = a.c_net[k];
+            if (active(a, j)) a.status[j] = 2;
+        }
+    }
+}
+
+// 6. end of the round.  The batch goes on after a commit below the round
+//    limit: cells of blocked nets are released.  Otherwise the batch ends:
+//    each cell's list moves past assigned nets, the next batch starts from
+//    the kept map with every unassigned net active.  The successor lists go
+//    to a buffer neither current list uses.  Thread 0 writes the successor
+//    control state.
+__global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t cells) {
+    const int* st = a.ctrl + (g & 1) * C_WORDS;
+    int* nx = a.ctrl + ((g + 1) & 1) * C_WORDS;
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * TPB;   // grid-stride over cells and list entries
+    const bool done = st[C_DONE] != 0;
+    const bool commit = st[C_COMMIT] != 0;
+    const int round = st[C_ROUND];
+    const bool end = !commit || round + 1 >= ROUNDS;
+    const int fresh = 1 - st[C_UL];   // the unassigned-list buffer not in use
+    int* nl = a.lists + (int64_t)fresh * a.n;
+    if (!done) {
+        if (end) {
+            for (int64_t q = t; q < cells; q += stride) {
+                int64_t h = a.head[q];
+                const int64_t e = a.c_off[q + 1];
+                while (h < e && a.batch[a.c_net[h]] != -1) h++;
+                a.head[q] = h;
+                const int b = h < e ? a.c_net[h] : -1;
+                a.base[q] = b;
+                a.map[q] = b;
+            }
+            const int* ul = a.lists + (int64_t)st[C_UL] * a.n;
+            for (int64_t u = t; u < st[C_NU]; u += stride) {
+                const int i = ul[u];
+                const bool left = a.batch[i] == -1;
+                a.status[i] = left ? 1 : 0;
+                if (left) nl[atomicAdd(&nx[C_NU], 1)] = i;
+            }
+        } else {
+            // UNR independent cells per lane and pass; the pass count is the
+            // same for every lane of the wave (wave_append needs all lanes)
+            constexpr int UNR = 4;
+            const int64_t per = stride * UNR;
+            const int64_t passes = (cells + per - 1) / per;
+            for (int64_t ps = 0; ps < passes; ps++) {
+                int m[UNR];
+                for (int u = 0; u < UNR; u++) {
+                    const int64_t q = ps * per + u * stride + t;
+                    m[u] = q < cells ? a.map[q] : -1;
+                }
+                int s[UNR];
+                for (int u = 0; u < UNR; u++) s[u] = m[u] >= 0 ? a.status[m[u]] : 0;
+                for (int u = 0; u < UNR; u++) {
+                    const int64_t q = ps * per + u * stride + t;
+                    const bool rl = s[u] == 2;
+                    if (rl) a.map[q] = -1;
+                    wave_append(rl, (int)q, &nx[C_NR], a.rel);
+                }
+            }
+        }
+    }
+    if (t == 0) {
+        if (!done) {
+            int64_t na;
+            (void)r_list(a, st, na);
+            a.stat[0] += 1;
+            a.stat[1] += (unsigned long long)na;
+            if (round == 0) a.stat[2] += (unsigned long long)st[C_NU];
+        }
+        int d = st[C_DONE], cur = st[C_CUR], r = st[C_ROUND], have = st[C_HAVE] | (commit ? 1 : 0);
+        if (!d) {
+            if (!end) {
+                r = round + 1;
+            } else if (!have) {
+                d = 1;                       // a batch without commits: nothing left to place
+            } else {
+                cur += 1;
+                r = 0;
+                have = 0;
+                if ((int64_t)a.ctrl[2 * C_WORDS] == a.n) d = 1;
+            }
+        }
+        nx[C_DONE] = d;
+        nx[C_CUR] = cur;
+        nx[C_ROUND] = r;
+        nx[C_HAVE] = have;
+        nx[C_COMMIT] = 0;
+        nx[C_NVIABLE] = 0;
+        nx[C_NK] = 0;
+        nx[C_NC] = 0;
+        nx[C_NH] = 0;
+        if (done) {
+            nx[C_NU] = st[C_NU];
+            nx[C_UL] = st[C_UL];
+            nx[C_NR] = st[C_NR];
+        } else if (end) {
+            nx[C_UL] = fresh;        // C_NU counted above
+        } else {
+            nx[C_NU] = st[C_NU];     // C_NR counted above
+            nx[C_UL] = st[C_UL];
+        }
+    }
+}
+
 __global__ void k_fill_b(int* p, int64_t n, int v) {
     const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (i < n) p[i] = v;
@@ -427,10 +729,10 @@ __global__ void k_start(BArgs a) {
     const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (i < a.n) a.lists[i] = (int)i;
     if (i == 0) {
-        a.ctrl[C_NA] = -1;
-        a.ctrl[C_AL] = 0;
         a.ctrl[C_NU] = (int)a.n;
         a.ctrl[C_UL] = 0;
+        a.ctrl[C_NA] = -1;   // active-list rounds: the unassigned list is the active list
+        a.ctrl[C_AL] = 0;
     }
 }
 
@@ -464,11 +766,24 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
     S.status.alloc(n);
     S.viable.alloc(n);
     S.lists.alloc(NLISTS * n);
+    // Release-driven rounds from this many nets (active-list rounds below).
+    const bool release = c->tune.sched_kind ? c->tune.sched_kind == 2 : n >= DEF_SCHED_RELEASE_NETS;
+    if (release) {
+        S.rel.alloc(XYc);
+        S.cand.alloc(n);
+        S.cstamp.zero(n, st);
+        S.comm.alloc(n);
+        S.hits.alloc(XYc);
+        S.hstamp.zero(XYc, st);
+    }
     S.stat.zero(3, st);
     S.ctrl.zero(2 * C_WORDS + 1, st);
     BArgs a{n, S.ids.p, d_pt_off ? S.pids.p : S.ids.p, d_pt_off ? d_pt_off : N.pt_off.p, d_pt_off ? d_pt : N.pt.p,
             N.hs_off.p, N.hs.p, N.vs_off.p, N.vs.p, S.map.p, S.base.p, S.c_off.p, nullptr, S.head.p,
-            S.batch.p, S.status.p, S.ctrl.p, S.lists.p, S.stat.p, c->g.X, c->g.Y,
+            S.batch.p, S.status.p, S.ctrl.p, S.lists.p,
+            release ? S.rel.p : nullptr, release ? S.cand.p : nullptr, release ? S.cstamp.p : nullptr,
+            release ? S.comm.p : nullptr, release ? S.hits.p : nullptr, release ? S.hstamp.p : nullptr, S.stat.p,
+            c->g.X, c->g.Y,
             (mode == 1 || mode == 2) ? 2 : 15};
     // Per-cell lists of the nets' point cells, highest priority first.
     k_cell_count<<<blocks(n), TPB, 0, st>>>(a, S.cnt.p);
@@ -514,19 +829,50 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
                          std::chrono::duration<double>(t_now - t_start).count());
             }
             for (int r = 0; r < CHUNK; r++, g++) {
-                switch (G) {
-                case 4: k_b_claim<4><<<ggrid, TPB, 0, st>>>(a, g); k_b_viable<4><<<ggrid, TPB, 0, st>>>(a, g, S.viable.p); break;
-                case 8: k_b_claim<8><<<ggrid, TPB, 0, st>>>(a, g); k_b_viable<8><<<ggrid, TPB, 0, st>>>(a, g, S.viable.p); break;
-                default: k_b_claim<16><<<ggrid, TPB, 0, st>>>(a, g); k_b_viable<16><<<ggrid, TPB, 0, st>>>(a, g, S.viable.p); break;
+                if (release) {
+                    switch (G) {
+                    case 4:
+                        k_r_claim<4><<<ggrid, TPB, 0, st>>>(a, g);
+                        k_r_viable<4><<<ggrid, TPB, 0, st>>>(a, g, S.viable.p);
+                        break;
+                    case 8:
+                        k_r_claim<8><<<ggrid, TPB, 0, st>>>(a, g);
+                        k_r_viable<8><<<ggrid, TPB, 0, st>>>(a, g, S.viable.p);
+                        break;
+                    default:
+                        k_r_claim<16><<<ggrid, TPB, 0, st>>>(a, g);
+                        k_r_viable<16><<<ggrid, TPB, 0, st>>>(a, g, S.viable.p);
+                        break;
+                    }
+                    k_b_claim_rest<<<wgrid, TPB, 0, st>>>(a, g, S.viable.p);
+                    k_r_commit<<<wgrid, TPB, 0, st>>>(a, g, S.viable.p);
+                    k_r_hits<<<wgrid, TPB, 0, st>>>(a, g);
+                    k_r_block<16><<<ggrid, TPB, 0, st>>>(a, g);
+                    k_r_clear_end<<<gm, TPB, 0, st>>>(a, g, XYc);
+                } else {
+                    switch (G) {
+                    case 4:
+                        k_a_claim<4><<<ggrid, TPB, 0, st>>>(a, g);
+                        k_a_viable<4><<<ggrid, TPB, 0, st>>>(a, g, S.viable.p);
+                        break;
+                    case 8:
+                        k_a_claim<8><<<ggrid, TPB, 0, st>>>(a, g);
+                        k_a_viable<8><<<ggrid, TPB, 0, st>>>(a, g, S.viable.p);
+                        break;
+                    default:
+                        k_a_claim<16><<<ggrid, TPB, 0, st>>>(a, g);
+                        k_a_viable<16><<<ggrid, TPB, 0, st>>>(a, g, S.viable.p);
+                        break;
+                    }
+                    k_b_claim_rest<<<wgrid, TPB, 0, st>>>(a, g, S.viable.p);
+                    k_a_commit<<<wgrid, TPB, 0, st>>>(a, g, S.viable.p);
+                    switch (G) {
+                    case 4: k_a_revert<4><<<ggrid, TPB, 0, st>>>(a, g); break;
+                    case 8: k_a_revert<8><<<ggrid, TPB, 0, st>>>(a, g); break;
+                    default: k_a_revert<16><<<ggrid, TPB, 0, st>>>(a, g); break;
+                    }
+                    k_a_clear_end<<<gm, TPB, 0, st>>>(a, g, XYc);
                 }
-                k_b_claim_rest<<<wgrid, TPB, 0, st>>>(a, g, S.viable.p);
-                k_b_commit<<<wgrid, TPB, 0, st>>>(a, g, S.viable.p);
-                switch (G) {
-                case 4: k_b_revert<4><<<ggrid, TPB, 0, st>>>(a, g); break;
-                case 8: k_b_revert<8><<<ggrid, TPB, 0, st>>>(a, g); break;
-                default: k_b_revert<16><<<ggrid, TPB, 0, st>>>(a, g); break;
-                }
-                k_b_clear_end<<<gm, TPB, 0, st>>>(a, g, XYc);
             }
             HIP_OK(hipGetLastError());
             HIP_OK(hipMemcpyAsync(h_ctrl, S.ctrl.p, sizeof(int) * (2 * C_WORDS + 1), hipMemcpyDeviceToHost, st));
@@ -543,8 +889,9 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
         unsigned long long sv[3] = {0, 0, 0};
         HIP_OK(hipMemcpyAsync(sv, S.stat.p, sizeof(sv), hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
-        progress("scheduler: %d batches, %llu rounds, %.0f active nets per round, %.0f unassigned per batch", nbatch,
-                 sv[0], sv[0] ? (double)sv[1] / (double)sv[0] : 0.0, nbatch ? (double)sv[2] / nbatch : 0.0);
+        progress("scheduler (%s rounds): %d batches, %llu rounds, %.0f nets tested per round, %.0f unassigned per "
+                 "batch", release ? "release-driven" : "active-list", nbatch, sv[0],
+                 sv[0] ? (double)sv[1] / (double)sv[0] : 0.0, nbatch ? (double)sv[2] / nbatch : 0.0);
     }
     std::vector<int> b(n);
     HIP_OK(hipMemcpyAsync(b.data(), S.batch.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));

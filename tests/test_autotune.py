@@ -30,7 +30,7 @@ def test_tune_small_design_keeps_routes():
     res = autotune.tune(d, rrr=1, threads=8,
                         candidates={"sched_group": [4, 16], "sched_chunk": [4, 32], "graph": [1, 2],
                                     "dag_order": [1, 2], "dag_waves": [2, 4], "track_k": [4, 8],
-                                    "dag_edges": [1, 2], "dag_seq": [2, 1 << 30]})
+                                    "dag_edges": [1, 2], "dag_seq": [2, 1 << 30], "sched_kind": [1, 2]})
     t = res["tuning"]
     assert t["sched_group"] in (4, 16) and t["sched_chunk"] in (4, 32)
     assert t["graph"] in (1, 2) and t["dag_order"] in (1, 2) and t["dag_waves"] in (2, 4)

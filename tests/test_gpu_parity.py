@@ -70,11 +70,14 @@ def test_presum_track_lengths(shape):
     r.close()
 
 
-@pytest.mark.parametrize("form", [gpu.FORM_PLAIN, gpu.FORM_BATCHED], ids=["plain", "batched"])
+@pytest.mark.parametrize("form,kind", [(gpu.FORM_PLAIN, 0), (gpu.FORM_BATCHED, 1), (gpu.FORM_BATCHED, 2)],
+                         ids=["plain", "batched-active-list", "batched-release-driven"])
 @pytest.mark.parametrize("mode", [0, 1])
-def test_schedule_bitexact(design_c1, form, mode):
+def test_schedule_bitexact(design_c1, form, kind, mode):
     d = design_c1
     r = gpu.Router.from_design(d)
+    if kind:
+        r.set_tuning(sched_kind=kind)
     o = orc.Oracle.from_design(d)
     r.set_nets_from_design(d)
     o.set_nets_from_design(d)

@@ -18,10 +18,13 @@ def _routes(h):
     return [data[off[i]:off[i] + data[off[i]]] for i in range(len(off) - 1)]
 
 
-@pytest.mark.parametrize("form", [gpu.FORM_PLAIN, gpu.FORM_BATCHED], ids=["plain", "batched"])
-def test_rrr_three_passes_c1(design_c1, form):
+@pytest.mark.parametrize("form,kind", [(gpu.FORM_PLAIN, 0), (gpu.FORM_BATCHED, 0), (gpu.FORM_BATCHED, 2)],
+                         ids=["plain", "batched", "batched-release-driven"])
+def test_rrr_three_passes_c1(design_c1, form, kind):
     d = design_c1
     r = gpu.Router.from_design(d)
+    if kind:
+        r.set_tuning(sched_kind=kind)
     o = orc.Oracle.from_design(d)
     r.set_nets_from_design(d)
     o.set_nets_from_design(d)
