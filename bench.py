@@ -56,6 +56,9 @@ def parse():
     ap.add_argument("--autotune-nets", type=int, default=0,
                     help="before timing, tune the batched launch shapes on a sample design of this many nets "
                          "from the same generator (untimed)")
+    ap.add_argument("--host-score", action="store_true",
+                    help="also score the written route file with the host evaluator on designs above "
+                         "configuration 1 (always done up to configuration 1)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (bandwidth reference, counter calibration kernel)")
     ap.add_argument("--summarize-stats", metavar="DIR",
@@ -386,6 +389,12 @@ def main():
             last = (r, st)
         r, st = last
         routes[f] = r.routes_flat()
+        # Score of the routes on the device (GPU evaluator), before the
+        # context goes.
+        t_ev = time.perf_counter()
+        gscore = r.evaluate(design)
+        gscore["eval_s"] = time.perf_counter() - t_ev
+        _progress("%s GPU evaluator: %s" % (f, json.dumps(gscore)))
         if not a.no_profile:
             # Kernel timing (events on the stream, no graph) in one extra,
             # untimed step.
@@ -397,7 +406,7 @@ def main():
                 raise RuntimeError("profiled step routed differently")
         kt = r.kernel_times()
         kb = r.kernel_bytes()
-        results[f] = {"step_s": times, "stats": st, "kernels": kt, "bytes": kb}
+        results[f] = {"step_s": times, "stats": st, "kernels": kt, "bytes": kb, "gpu_score": gscore}
         print("[bench] %s: %s" % (f, json.dumps({"step_s": times, "pattern_batches": st["pattern_batches"],
                                                    "ripped": st["ripped"], "rrr_batches": st["rrr_batches"],
                                                    "tree_s": st["tree_s"],
@@ -414,18 +423,30 @@ def main():
 
     info = design.info
     nets_in = info.nets
-    # Identical routes and score across forms.
-    import numpy as np
+    # Identical routes and score across forms.  The score is the GPU
+    # evaluator's; the host evaluator scores the written route file beside it
+    # (the check that the file the command line writes scores the same),
+    # skipped on designs above configuration 1 unless asked for.
     same = None
-    scores = {}
-    with tempfile.TemporaryDirectory() as tmp:
-        for f in forms:
-            off, data = routes[f]
-            p = os.path.join(tmp, f + ".out")
-            design.write_routes(p, off, data)
-            scores[f] = design.evaluate(p)
+    scores = {f: results[f]["gpu_score"] for f in forms}
+    if a.host_score or a.config <= 1:
+        with tempfile.TemporaryDirectory() as tmp:
+            for f in forms:
+                off, data = routes[f]
+                p = os.path.join(tmp, f + ".out")
+                t_h = time.perf_counter()
+                design.write_routes(p, off, data)
+                h = design.evaluate(p)
+                h["write_eval_s"] = time.perf_counter() - t_h
+                g = scores[f]
+                if h["opens"] != g["opens"] or abs(h["total"] - g["total"]) > 1e-9 * abs(h["total"]):
+                    raise RuntimeError("GPU and host evaluators disagree: %s vs %s" % (g, h))
+                scores[f] = dict(g, host=h)
     if len(forms) == 2:
         same = gpu.routes_equal(routes[forms[0]], routes[forms[1]])
+        g0, g1 = (results[f]["gpu_score"] for f in forms)
+        if same and (g0["opens"] != g1["opens"] or abs(g0["total"] - g1["total"]) > 1e-9 * abs(g1["total"])):
+            raise RuntimeError("identical routes scored differently")
     sweeps = cost_prefix_sweeps(design)
     cli = None
     if a.cli:

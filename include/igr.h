@@ -152,6 +152,39 @@ int igr_get_route(igr_ctx* ctx, int net, int* buf, int cap, int* len);
 /* All routes packed: off[n+1] (host) where off[i+1]-off[i] is net i's slot
  * capacity, data with data_cap >= igr_routes_capacity() entries. */
 int igr_get_routes(igr_ctx* ctx, int64_t* off, int* data, int64_t data_cap);
+
+/* GPU evaluator: connectivity check and FP64 score (wirelength, vias,
+ * overflow) with the definitions of run/evaluator.cpp:458-633,635-675, computed
+ * from routes in device memory instead of a written and parsed route file
+ * (the reference scores with the separate evaluator program,
+ * run/evaluator.cpp:279-301).  The route file's lines are rebuilt as the
+ * writer makes them (src/graph.hpp:179-235): per net its metal-1 pin vias,
+ * its stacked-pin via pairs and its subnets' routes one layer up.  Unlike the
+ * reference program, an open net is counted instead of ending the run.
+ * route_off / routes: routes in igr_get_routes' layout, or NULL for the
+ * routes the context holds.  Metal layers are routing layers + 1; cells
+ * z*X*Y + x*Y + y. */
+typedef struct igr_eval_nets {
+    int64_t nets;
+    const int64_t* sub_off;      /* nets+1: each net's subnets (context net indices) */
+    const int* subs;
+    const int64_t* pin_off;      /* nets+1: selected pin cells; a cell on metal 1 gets a via to metal 2 */
+    const int* pins;
+    const int64_t* extra_off;    /* nets+1: stacked-pin via pairs (metal-layer cells) */
+    const int* extra;
+    const int64_t* dpin_off;     /* nets+1: design pins */
+    const int64_t* ap_off;       /* design pins+1: access points (metal-layer cells) */
+    const int* ap;
+    const int* dir;              /* per metal layer: 0 horizontal, 1 vertical */
+    const double* of_weight;     /* per metal layer */
+    const double* capacity;      /* metal layers x X x Y */
+} igr_eval_nets;
+typedef struct igr_score {
+    double wirelength, via, overflow, total;
+    int64_t opens;
+} igr_score;
+int igr_evaluate(igr_ctx* ctx, const igr_eval_nets* nets, const int64_t* route_off, const int* routes,
+                 igr_score* out);
 int64_t igr_routes_capacity(igr_ctx* ctx);
 
 /* graph::ripup's marking (src/graph.hpp:588-601): is_of[net] = 1 when more

@@ -412,6 +412,24 @@ int64_t igr_routes_capacity(igr_ctx* c) {
     return t;
 }
 
+namespace igr_dev {
+void gather_routes(igr_ctx* c, const int64_t* d_off, int* d_out) {
+    const int64_t n = c->nets.n;
+    if (n) k_gather_routes<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(c->nets.rptr.p, d_off, d_out, n);
+    HIP_OK(hipGetLastError());
+}
+} // namespace igr_dev
+
+int igr_evaluate(igr_ctx* c, const igr_eval_nets* e, const int64_t* route_off, const int* routes, igr_score* out) {
+    return guard([&]() {
+        if (!e || !out) throw std::invalid_argument("null argument");
+        if ((route_off == nullptr) != (routes == nullptr))
+            throw std::invalid_argument("route_off and routes go together");
+        if (!route_off && c->nets.n == 0) throw std::invalid_argument("the context holds no routes");
+        evaluate_routes_dev(c, e, route_off, routes, out);
+    });
+}
+
 int igr_get_routes(igr_ctx* c, int64_t* off, int* data, int64_t cap) {
     return guard([&]() {
         const int64_t n = c->nets.n;
@@ -422,8 +440,7 @@ int igr_get_routes(igr_ctx* c, int64_t* off, int* data, int64_t cap) {
         DevBuf<int> out;
         doff.upload(o.data(), n + 1, c->stream);
         out.alloc(o[n]);
-        if (n) k_gather_routes<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(c->nets.rptr.p, doff.p, out.p, n);
-        HIP_OK(hipGetLastError());
+        gather_routes(c, doff.p, out.p);
         HIP_OK(hipMemcpyAsync(data, out.p, sizeof(int) * o[n], hipMemcpyDeviceToHost, c->stream));
         HIP_OK(hipStreamSynchronize(c->stream));
         std::memcpy(off, o.data(), sizeof(int64_t) * (n + 1));

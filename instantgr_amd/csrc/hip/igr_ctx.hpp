@@ -235,7 +235,7 @@ struct igr_ctx {
     igr_dev::Scratch scratch;
     igr_dev::DagDev dag;
     int stamp = 0;
-    igr_tuning tune{0, 0, 0, 0, 0, 0, 0, 0};   // launch-shape overrides (0 = built-in)
+    igr_tuning tune{0, 0, 0, 0, 0, 0, 0, 0, 0};   // launch-shape overrides (0 = built-in)
     double last_dag_prep_s = 0;   // host preparation time of the last DAG pass
     double last_dag_prep_parts[4] = {0, 0, 0, 0};   // fill, batch metadata, upload, allocation
     int64_t cells() const { return (int64_t)g.L * g.X * g.Y; }
@@ -363,5 +363,10 @@ struct BatchedState {
 // the next one (s_rip > 0), then costs and prefix sums when 'cost' is set.
 void launch_track_fused(igr_ctx* c, int s_commit, int s_rip, bool cost);
 double time_cost_sweep(igr_ctx* c, int form, int reps);
+// Every net's route into d_out at d_off[i] (slices of the nets' route capacities).
+void gather_routes(igr_ctx* c, const int64_t* d_off, int* d_out);
+// The GPU evaluator (eval_kernels.hip).
+void evaluate_routes_dev(igr_ctx* c, const igr_eval_nets* e, const int64_t* h_route_off, const int* h_routes,
+                         igr_score* out);
 void launch_rip_scatter(igr_ctx* c, const int* d_nets, int64_t n);
 } // namespace igr_dev

@@ -121,6 +121,7 @@ int64_t igrh_array(const igrh_design* h, const char* name, const void** ptr) {
     A("net_sub_off", s.net_sub_off) A("net_sub", s.net_sub)
     A("pin_off", h->np.pin_off) A("pins", h->np.pins) A("extra_off", h->np.extra_off) A("extra", h->np.extra)
     A("ap_off", h->d.ap_off) A("ap", h->d.ap) A("net_pin_off", h->d.pin_off)
+    A("metal_cap", h->d.cap) A("metal_dir", h->d.dir) A("metal_of_weight", h->d.of_weight)
     A("dag_node_off", h->dags.node_off) A("dag_node", h->dags.node) A("dag_par_off", h->dags.par_off)
     A("dag_par", h->dags.par) A("dag_slot", h->dags.slot) A("dag_child_num", h->dags.child_num)
     A("dag_depth", h->dags.depth) A("dag_pt_off", h->dags.pt_off) A("dag_pt", h->dags.pt)

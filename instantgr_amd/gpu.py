@@ -53,6 +53,18 @@ class Tuning(C.Structure):
 TUNING_FIELDS = [f for f, _ in Tuning._fields_]
 
 
+class EvalNets(C.Structure):
+    _fields_ = [("nets", C.c_int64), ("sub_off", C.c_void_p), ("subs", C.c_void_p), ("pin_off", C.c_void_p),
+                ("pins", C.c_void_p), ("extra_off", C.c_void_p), ("extra", C.c_void_p), ("dpin_off", C.c_void_p),
+                ("ap_off", C.c_void_p), ("ap", C.c_void_p), ("dir", C.c_void_p), ("of_weight", C.c_void_p),
+                ("capacity", C.c_void_p)]
+
+
+class Score(C.Structure):
+    _fields_ = [("wirelength", C.c_double), ("via", C.c_double), ("overflow", C.c_double), ("total", C.c_double),
+                ("opens", C.c_int64)]
+
+
 class ServeOpts(C.Structure):
     _fields_ = [("threads", C.c_int), ("max_tick_nets", C.c_int64), ("scratch_bytes", C.c_int64),
                 ("hold_ms", C.c_int), ("overtake", C.c_int)]
@@ -98,6 +110,7 @@ SIGNATURES = {
                                C.POINTER(PassStats)]),
     "igr_get_route": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "igr_get_routes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
+    "igr_evaluate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "igr_routes_capacity": (C.c_int64, [C.c_void_p]),
     "igr_ripup_mark": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "igr_congestion_view": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -292,6 +305,32 @@ class Router:
         data = np.empty(max(cap, 1), np.int32)
         _check(lib().igr_get_routes(self.h, off.ctypes.data, data.ctypes.data, data.size))
         return off, data
+
+    def evaluate(self, design, routes=None):
+        """Score of the routes on the device (igr_evaluate): the routes the
+        context holds, or routes=(off, data) in routes_flat's layout.  The
+        same definitions as design.evaluate(route_file)."""
+        keep = []
+
+        def ptr(name, dt):
+            a = np.ascontiguousarray(design.array(name), dt)
+            keep.append(a)
+            return a.ctypes.data
+
+        e = EvalNets(design.info.nets, ptr("net_sub_off", np.int64), ptr("net_sub", np.int32),
+                     ptr("pin_off", np.int64), ptr("pins", np.int32), ptr("extra_off", np.int64),
+                     ptr("extra", np.int32), ptr("net_pin_off", np.int64), ptr("ap_off", np.int64),
+                     ptr("ap", np.int32), ptr("metal_dir", np.int32), ptr("metal_of_weight", np.float64),
+                     ptr("metal_cap", np.float64))
+        s = Score()
+        if routes is None:
+            _check(lib().igr_evaluate(self.h, C.byref(e), None, None, C.byref(s)))
+        else:
+            off = np.ascontiguousarray(routes[0], np.int64)
+            data = np.ascontiguousarray(routes[1], np.int32)
+            _check(lib().igr_evaluate(self.h, C.byref(e), off.ctypes.data, data.ctypes.data, C.byref(s)))
+        return {"wirelength": s.wirelength, "via": s.via, "overflow": s.overflow, "total": s.total,
+                "opens": int(s.opens)}
 
     def ripup_mark(self, th=0):
         out = np.empty(self.num_nets, np.uint8)

@@ -67,7 +67,8 @@ def _check(rc):
 _DTYPES = {"node_off": np.int64, "pt_off": np.int64, "hs_off": np.int64, "vs_off": np.int64,
            "sub_pin_off": np.int64, "tree_node_off": np.int64, "tree_adj_off": np.int64, "net_sub_off": np.int64,
            "pin_off": np.int64, "extra_off": np.int64, "ap_off": np.int64, "net_pin_off": np.int64,
-           "dag_node_off": np.int64, "dag_par_off": np.int64, "dag_pt_off": np.int64}
+           "dag_node_off": np.int64, "dag_par_off": np.int64, "dag_pt_off": np.int64,
+           "metal_cap": np.float64, "metal_of_weight": np.float64}
 
 
 def gen_params(layers=10, xsize=64, ysize=64, nets=500, pin_cap=30, geom_p=0.35, sigma=8.0,
@@ -126,7 +127,7 @@ class Design:
             if n == 0:
                 self._arrays[name] = np.zeros(0, dtype=dt)
             else:
-                ct = C.c_int64 if dt == np.int64 else C.c_int32
+                ct = {np.int64: C.c_int64, np.float64: C.c_double}.get(dt, C.c_int32)
                 self._arrays[name] = np.ctypeslib.as_array(C.cast(p, C.POINTER(ct)), shape=(n,))
         return self._arrays[name]
 
