@@ -37,7 +37,9 @@ namespace igr_dev {
 namespace {
 
 constexpr int ETPB = 256;
-constexpr int MAXML = 64;         // metal layers the per-block via counters hold
+constexpr int MAXML = 64;         // metal layers the per-block via counters and via masks hold
+constexpr int BIG_NET = 512;      // nets with more items: a workgroup each, items staged in LDS tiles
+constexpr int BTILE = 256;        // items per LDS tile (= threads of the workgroup)
 
 struct Item {
     int x0, x1, y0, y1, z0, z1;   // z0 > z1: dead
@@ -138,6 +140,38 @@ __device__ void uf_union(int* par, int x, int y) {
     }
 }
 
+// A counted via's lower cell (z, x, y): its demand is split between the cell
+// and its neighbour along the layer direction (run/evaluator.cpp:635-675).
+__device__ void add_svia(const EArgs& a, int x, int y, int z) {
+    const int X = a.X, Y = a.Y;
+    int* sv = a.svia + z * a.XY;
+    if (a.dir[z] == 0) {
+        if (x > 0 && x < X - 1) {
+            atomicAdd(&sv[(int64_t)(x - 1) * Y + y], 1);
+            atomicAdd(&sv[(int64_t)x * Y + y], 1);
+        } else if (x > 0) {
+            atomicAdd(&sv[(int64_t)(x - 1) * Y + y], 2);
+        } else if (x < X - 1) {
+            atomicAdd(&sv[(int64_t)x * Y + y], 2);
+        }
+    } else {
+        if (y > 0 && y < Y - 1) {
+            atomicAdd(&sv[(int64_t)x * Y + y - 1], 1);
+            atomicAdd(&sv[(int64_t)x * Y + y], 1);
+        } else if (y > 0) {
+            atomicAdd(&sv[(int64_t)x * Y + y - 1], 2);
+        } else if (y < Y - 1) {
+            atomicAdd(&sv[(int64_t)x * Y + y], 2);
+        }
+    }
+}
+
+// Bits lo..hi (lo <= hi < 64).
+__device__ __forceinline__ uint64_t bits(int lo, int hi) {
+    const int w = hi - lo + 1;
+    return (w >= 64 ? ~0ull : ((1ull << w) - 1)) << lo;
+}
+
 // Items per net: metal-1 pin vias, stacked-pin via pairs, route pairs.
 __global__ void k_ev_count(EArgs a, int64_t* cnt) {
     const int64_t n = (int64_t)blockIdx.x * ETPB + threadIdx.x;
@@ -202,6 +236,7 @@ __global__ void __launch_bounds__(ETPB) k_ev_net(EArgs a) {
          n += ((int64_t)gridDim.x * ETPB) >> 6) {
         const int64_t base = a.it_off[n];
         const int m = (int)(a.it_off[n + 1] - base);
+        if (m > BIG_NET) continue;   // k_ev_net_big (the same for the whole wave)
         const Item* it = a.items + base;
         int* par = a.par + base;
         int* tr = a.traced + base;
@@ -218,27 +253,7 @@ __global__ void __launch_bounds__(ETPB) k_ev_net(EArgs a) {
                     else if (is_wire(t) && t.z0 == z && contains(t, c.x0, c.y0, z)) skip = true;
                 }
                 if (skip) continue;
-                const int x = c.x0, y = c.y0;
-                int* sv = a.svia + z * a.XY;
-                if (a.dir[z] == 0) {
-                    if (x > 0 && x < X - 1) {
-                        atomicAdd(&sv[(int64_t)(x - 1) * Y + y], 1);
-                        atomicAdd(&sv[(int64_t)x * Y + y], 1);
-                    } else if (x > 0) {
-                        atomicAdd(&sv[(int64_t)(x - 1) * Y + y], 2);
-                    } else if (x < X - 1) {
-                        atomicAdd(&sv[(int64_t)x * Y + y], 2);
-                    }
-                } else {
-                    if (y > 0 && y < Y - 1) {
-                        atomicAdd(&sv[(int64_t)x * Y + y - 1], 1);
-                        atomicAdd(&sv[(int64_t)x * Y + y], 1);
-                    } else if (y > 0) {
-                        atomicAdd(&sv[(int64_t)x * Y + y - 1], 2);
-                    } else if (y < Y - 1) {
-                        atomicAdd(&sv[(int64_t)x * Y + y], 2);
-                    }
-                }
+                add_svia(a, c.x0, c.y0, z);
             }
         }
         const int64_t p0 = a.dpin_off[n], p1 = a.dpin_off[n + 1];
@@ -273,6 +288,102 @@ __global__ void __launch_bounds__(ETPB) k_ev_net(EArgs a) {
             ok = __ballot(hit) != 0;
         }
         if (lane == 0 && !ok) atomicAdd(a.opens, 1ull);
+    }
+}
+
+// A workgroup per net with more than BIG_NET items: the same work as
+// k_ev_net, with the item pairs compared through LDS tiles.  A thread keeps
+// one item (or one via column's counted-via mask) in registers and compares
+// it with each tile, so global loads grow with m * m / BTILE, not m * m.
+__global__ void __launch_bounds__(BTILE) k_ev_net_big(EArgs a, const int* big, int nbig) {
+    __shared__ Item tile[BTILE];
+    __shared__ int s_hit;
+    const int tid = threadIdx.x;
+    const int X = a.X, Y = a.Y;
+    const Item dead{0, 0, 0, 0, 1, 0};
+    for (int b = blockIdx.x; b < nbig; b += gridDim.x) {   // the same for the whole workgroup
+        const int n = big[b];
+        const int64_t base = a.it_off[n];
+        const int m = (int)(a.it_off[n + 1] - base);
+        const Item* it = a.items + base;
+        int* par = a.par + base;
+        int* tr = a.traced + base;
+        // Stacked vias: bit z of 'skip' drops the via on layer z of column i.
+        for (int c0 = 0; c0 < m; c0 += BTILE) {
+            const int i = c0 + tid;
+            const Item c = i < m ? it[i] : dead;
+            const bool col = c.z0 < c.z1;
+            uint64_t skip = 0;
+            for (int t0 = 0; t0 < m; t0 += BTILE) {
+                __syncthreads();
+                if (t0 + tid < m) tile[tid] = it[t0 + tid];
+                __syncthreads();
+                const int tn = min(BTILE, m - t0);
+                if (!col) continue;
+                for (int k = 0; k < tn; k++) {
+                    const Item t = tile[k];
+                    if (t.x0 > c.x0 || t.x1 < c.x0 || t.y0 > c.y0 || t.y1 < c.y0) continue;
+                    if (t.z0 < t.z1) {
+                        // an earlier column over the same gcell holds the vias t.z0..t.z1-1
+                        if (t0 + k < i) {
+                            const int lo = max(t.z0, c.z0), hi = min(t.z1, c.z1) - 1;
+                            if (lo <= hi) skip |= bits(lo, hi);
+                        }
+                    } else if (is_wire(t) && t.z0 >= c.z0 && t.z0 < c.z1) {
+                        skip |= 1ull << t.z0;   // a wire of the net on that layer
+                    }
+                }
+            }
+            if (col)
+                for (int z = max(c.z0, 1); z < c.z1; z++)
+                    if (!((skip >> z) & 1)) add_svia(a, c.x0, c.y0, z);
+        }
+        const int64_t p0 = a.dpin_off[n], p1 = a.dpin_off[n + 1];
+        if (p1 - p0 < 2) {
+            if (tid == 0 && a.bad[n]) atomicAdd(a.opens, 1ull);
+            continue;
+        }
+        // Components: item i against every later item, a tile at a time.
+        for (int c0 = 0; c0 < m; c0 += BTILE) {
+            const int i = c0 + tid;
+            const Item s = i < m ? it[i] : dead;
+            for (int t0 = c0; t0 < m; t0 += BTILE) {
+                __syncthreads();
+                if (t0 + tid < m) tile[tid] = it[t0 + tid];
+                __syncthreads();
+                const int tn = min(BTILE, m - t0);
+                if (i >= m) continue;
+                for (int k = 0; k < tn; k++)
+                    if (t0 + k > i && joined(a, s, tile[k])) uf_union(par, i, t0 + k);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+        __syncthreads();
+        for (int64_t k = a.ap_off[p0]; k < a.ap_off[p0 + 1]; k++) {
+            const int v = a.ap[k];
+            const int z = (int)(v / a.XY), x = v / Y % X, y = v % Y;
+            for (int j = tid; j < m; j += BTILE)
+                if (contains(it[j], x, y, z)) atomicExch(&tr[uf_find(par, j)], 1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+        __syncthreads();
+        bool ok = !a.bad[n];
+        for (int64_t p = p0 + 1; p < p1 && ok; p++) {
+            if (tid == 0) s_hit = 0;
+            __syncthreads();
+            bool hit = false;
+            for (int64_t k = a.ap_off[p]; k < a.ap_off[p + 1] && !hit; k++) {
+                const int v = a.ap[k];
+                const int z = (int)(v / a.XY), x = v / Y % X, y = v % Y;
+                for (int j = tid; j < m && !hit; j += BTILE)
+                    hit = contains(it[j], x, y, z) && uf_load(&tr[uf_find(par, j)]) != 0;
+            }
+            if (hit) s_hit = 1;
+            __syncthreads();
+            ok = s_hit != 0;
+            __syncthreads();
+        }
+        if (tid == 0 && !ok) atomicAdd(a.opens, 1ull);
     }
 }
 
@@ -429,6 +540,14 @@ void evaluate_routes_dev(igr_ctx* c, const igr_eval_nets* e, const int64_t* h_ro
     k_ev_fill<<<(unsigned)((NN + ETPB - 1) / ETPB), ETPB, 0, st>>>(a);
     k_ev_wires<<<eblocks(total, 4096), ETPB, 0, st>>>(a, total);
     k_ev_net<<<eblocks(NN * 64, 8192), ETPB, 0, st>>>(a);
+    std::vector<int> hbig;
+    for (int64_t n = 0; n < NN; n++)
+        if (hc[n] > BIG_NET) hbig.push_back((int)n);
+    DevBuf<int> big;
+    if (!hbig.empty()) {
+        big.upload(hbig.data(), (int64_t)hbig.size(), st);
+        k_ev_net_big<<<(unsigned)std::min<size_t>(hbig.size(), 4096), BTILE, 0, st>>>(a, big.p, (int)hbig.size());
+    }
     const unsigned sb = eblocks((ML - 1) * XY, 2048);
     part_of.alloc(sb);
     part_wl.alloc(sb);

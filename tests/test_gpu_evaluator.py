@@ -87,3 +87,30 @@ def test_rejects_bad_route_slices(routed):
     data[off[0]] = 2        # an odd number of cells
     with pytest.raises(RuntimeError):
         r.evaluate(d, (off, data))
+
+
+def test_heavy_nets_score_as_the_route_file():
+    # The configuration-2 generator at 20k nets: its 100-1000-pin nets have
+    # more route lines than one wave takes, so they go through the
+    # workgroup-per-net kernel.
+    from instantgr_amd.host import Design
+    d = Design.preset(2, nets=20000).build(8)
+    r = gpu.Router.from_design(d)
+    r.run(d, form=gpu.FORM_BATCHED, rrr=1, threads=8)
+    off, data = r.routes_flat()
+    net_sub_off, net_sub = d.array("net_sub_off"), d.array("net_sub")
+    lines = np.array([sum((data[off[s]] - 1) // 2 for s in net_sub[net_sub_off[n]:net_sub_off[n + 1]])
+                      for n in range(d.info.nets)])
+    heavy = np.nonzero(lines > 600)[0]
+    assert len(heavy) >= 10
+    h = _host_score(d, off, data)
+    _same(r.evaluate(d), h)
+    # drop one subnet route of 5 heavy nets
+    data = data.copy()
+    for n in heavy[:5]:
+        s = net_sub[net_sub_off[n] + 1]
+        data[off[s]] = 1
+    h = _host_score(d, off, data)
+    assert h["opens"] >= 1
+    _same(r.evaluate(d, (off, data)), h)
+    r.close()
