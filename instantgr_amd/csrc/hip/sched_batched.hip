@@ -233,17 +233,27 @@ __device__ __forceinline__ uint64_t group_bits(uint64_t m, int G) {
     return (m >> base) & mask;
 }
 
-// Appends v to list for every lane with want, one counter atomic per wave.
-// Every lane of the wave calls it.
-__device__ __forceinline__ void wave_append(bool want, int v, int* counter, int* list) {
+// Appends v to list for every thread with want, one counter atomic per
+// workgroup (a returning atomic per wave on one counter serialises thousands
+// of waves).  Every thread of the workgroup calls it; s holds TPB/64+1 ints.
+__device__ __forceinline__ void block_append(bool want, int v, int* counter, int* list, int* s) {
+    constexpr int NW = TPB / 64;
     const uint64_t m = __ballot(want);
-    if (m == 0) return;
-    const int lane = threadIdx.x & 63;
-    const int leader = __builtin_ctzll(m);
-    int base = 0;
-    if (lane == leader) base = atomicAdd(counter, __popcll(m));
-    base = __shfl(base, leader);
-    if (want) list[base + __popcll(m & ((1ull << lane) - 1))] = v;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) s[wv] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < NW; w++) {
+            const int c = s[w];
+            s[w] = tot;
+            tot += c;
+        }
+        s[NW] = tot ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
+    if (want) list[s[NW] + s[wv] + __popcll(m & ((1ull << lane) - 1))] = v;
+    __syncthreads();   // s is reused by the next call
 }
 
 // 3. (both) a wave per viable net claims the neighbour cells (0,-1), (-1,0),
@@ -383,6 +393,7 @@ __global__ void __launch_bounds__(TPB) k_a_revert(BArgs a, int64_t g) {
 //    to a buffer neither current list uses.  Thread 0 writes the successor
 //    control state.
 __global__ void __launch_bounds__(TPB) k_a_clear_end(BArgs a, int64_t g, int64_t cells) {
+    __shared__ int s_app[TPB / 64 + 1];
     const int* st = a.ctrl + (g & 1) * C_WORDS;
     int* nx = a.ctrl + ((g + 1) & 1) * C_WORDS;
     const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
@@ -406,11 +417,13 @@ __global__ void __launch_bounds__(TPB) k_a_clear_end(BArgs a, int64_t g, int64_t
                 a.map[q] = b;
             }
             const int* ul = a.lists + (int64_t)st[C_UL] * a.n;
-            for (int64_t u = t; u < st[C_NU]; u += stride) {
-                const int i = ul[u];
-                const bool left = a.batch[i] == -1;
-                a.status[i] = left ? 1 : 0;
-                if (left) nl[atomicAdd(&nx[C_NU], 1)] = i;
+            const int64_t nu = st[C_NU], passes = (nu + stride - 1) / stride;   // the same for every thread
+            for (int64_t ps = 0; ps < passes; ps++) {
+                const int64_t u = ps * stride + t;
+                const int i = u < nu ? ul[u] : 0;
+                const bool left = u < nu && a.batch[i] == -1;
+                if (u < nu) a.status[i] = left ? 1 : 0;
+                block_append(left, i, &nx[C_NU], nl, s_app);
             }
         } else {
             for (int64_t q = t; q < cells; q += stride) {
@@ -419,9 +432,11 @@ __global__ void __launch_bounds__(TPB) k_a_clear_end(BArgs a, int64_t g, int64_t
             }
             int64_t na;
             const int* al = a_list(a, st, na);
-            for (int64_t u = t; u < na; u += stride) {
-                const int i = al[u];
-                if (active(a, i)) nl[atomicAdd(&nx[C_NA], 1)] = i;
+            const int64_t passes = (na + stride - 1) / stride;   // the same for every thread
+            for (int64_t ps = 0; ps < passes; ps++) {
+                const int64_t u = ps * stride + t;
+                const int i = u < na ? al[u] : 0;
+                block_append(u < na && active(a, i), i, &nx[C_NA], nl, s_app);
             }
         }
     }
@@ -476,12 +491,13 @@ __global__ void __launch_bounds__(TPB) k_a_clear_end(BArgs a, int64_t g, int64_t
 //    G entries a step.
 template <int G>
 __global__ void __launch_bounds__(TPB) k_r_claim(BArgs a, int64_t g) {
+    __shared__ int s_app[TPB / 64 + 1];
     int* st = a.ctrl + (g & 1) * C_WORDS;
     if (st[C_DONE] || st[C_ROUND] == 0) return;
     const int64_t nr = st[C_NR];
     const int64_t nth = (int64_t)gridDim.x * TPB / G;
     const int lg = threadIdx.x & (G - 1);
-    const int64_t rounds = (nr + nth - 1) / nth;   // the same for every lane of the wave
+    const int64_t rounds = (nr + nth - 1) / nth;   // the same for every thread
     for (int64_t r = 0; r < rounds; r++) {
         const int64_t t = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G + r * nth;
         const bool live = t < nr;
@@ -489,6 +505,8 @@ __global__ void __launch_bounds__(TPB) k_r_claim(BArgs a, int64_t g) {
         int64_t h = live ? a.head[q] : 0;
         const int64_t e = live ? a.c_off[q + 1] : 0;
         bool found = !live;
+        bool fresh = false;   // this lane took the cell for a net not yet listed this round
+        int owner = 0;
         for (;;) {
             const int64_t k = h + lg;
             int j = -1;
@@ -498,17 +516,16 @@ __global__ void __launch_bounds__(TPB) k_r_claim(BArgs a, int64_t g) {
                 act = active(a, j);
             }
             const uint64_t m = group_bits(__ballot(act), G);
-            const bool mine = m != 0 && lg == __builtin_ctzll(m);
-            bool fresh = false;
-            if (mine) {
+            if (m != 0 && lg == __builtin_ctzll(m)) {
                 a.map[q] = j;
+                owner = j;
                 fresh = atomicExch(&a.cstamp[j], (int)g + 1) != (int)g + 1;
             }
-            wave_append(fresh, j, &st[C_NK], a.cand);
             found = found || m != 0;
             h += G;
             if (__ballot(!found && h < e) == 0) break;
         }
+        block_append(fresh, owner, &st[C_NK], a.cand, s_app);
     }
 }
 
@@ -625,6 +642,7 @@ __global__ void __launch_bounds__(TPB) k_r_block(BArgs a, int64_t g) {
 //    to a buffer neither current list uses.  Thread 0 writes the successor
 //    control state.
 __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t cells) {
+    __shared__ int s_app[TPB / 64 + 1];
     const int* st = a.ctrl + (g & 1) * C_WORDS;
     int* nx = a.ctrl + ((g + 1) & 1) * C_WORDS;
     const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
@@ -647,15 +665,17 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
                 a.map[q] = b;
             }
             const int* ul = a.lists + (int64_t)st[C_UL] * a.n;
-            for (int64_t u = t; u < st[C_NU]; u += stride) {
-                const int i = ul[u];
-                const bool left = a.batch[i] == -1;
-                a.status[i] = left ? 1 : 0;
-                if (left) nl[atomicAdd(&nx[C_NU], 1)] = i;
+            const int64_t nu = st[C_NU], passes = (nu + stride - 1) / stride;   // the same for every thread
+            for (int64_t ps = 0; ps < passes; ps++) {
+                const int64_t u = ps * stride + t;
+                const int i = u < nu ? ul[u] : 0;
+                const bool left = u < nu && a.batch[i] == -1;
+                if (u < nu) a.status[i] = left ? 1 : 0;
+                block_append(left, i, &nx[C_NU], nl, s_app);
             }
         } else {
             // UNR independent cells per lane and pass; the pass count is the
-            // same for every lane of the wave (wave_append needs all lanes)
+            // same for every thread (block_append needs all of them)
             constexpr int UNR = 4;
             const int64_t per = stride * UNR;
             const int64_t passes = (cells + per - 1) / per;
@@ -671,7 +691,7 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
                     const int64_t q = ps * per + u * stride + t;
                     const bool rl = s[u] == 2;
                     if (rl) a.map[q] = -1;
-                    wave_append(rl, (int)q, &nx[C_NR], a.rel);
+                    block_append(rl, (int)q, &nx[C_NR], a.rel, s_app);
                 }
             }
         }
