@@ -110,7 +110,7 @@ struct Sched {
     // compacted unassigned / active net lists; release-driven rounds: the
     // released cells, new owners, committed nets and their cells, with stamps
     DevBuf<int> base, cnt, c_net, ctrl, lists, rel, cand, cstamp, comm, hits, hstamp;
-    DevBuf<int64_t> head, c_off, bsum;
+    DevBuf<int64_t> head, c_off, bsum, cur;
     DevBuf<unsigned long long> stat;
 };
 

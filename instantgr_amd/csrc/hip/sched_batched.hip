@@ -84,6 +84,7 @@ struct BArgs {
     int* comm;                    // nets committed this round
     int* hits;                    // X*Y cells owned by the nets committed this round
     int* hstamp;                  // per cell: 1 + the round that listed it in hits
+    int64_t* cur;                 // X*Y: no active net before this entry of the cell's list in this batch
     unsigned long long* stat;     // rounds run, nets tested for viability, unassigned at batch starts
     int X, Y, sparsity;
 };
@@ -152,6 +153,7 @@ __global__ void k_cell_sort(BArgs a, int64_t cells) {
         sift_min(v, 0, e);
     }
     a.head[q] = b;
+    if (a.cur) a.cur[q] = b;
     const int top = m > 0 ? v[0] : -1;
     a.base[q] = top;
     a.map[q] = top;
@@ -502,7 +504,7 @@ __global__ void __launch_bounds__(TPB) k_r_claim(BArgs a, int64_t g) {
         const int64_t t = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G + r * nth;
         const bool live = t < nr;
         const int q = live ? a.rel[t] : 0;
-        int64_t h = live ? a.head[q] : 0;
+        int64_t h = live ? a.cur[q] : 0;
         const int64_t e = live ? a.c_off[q + 1] : 0;
         bool found = !live;
         bool fresh = false;   // this lane took the cell for a net not yet listed this round
@@ -518,6 +520,7 @@ __global__ void __launch_bounds__(TPB) k_r_claim(BArgs a, int64_t g) {
             const uint64_t m = group_bits(__ballot(act), G);
             if (m != 0 && lg == __builtin_ctzll(m)) {
                 a.map[q] = j;
+                a.cur[q] = k;   // the entries before k stay inactive for the rest of the batch
                 owner = j;
                 fresh = atomicExch(&a.cstamp[j], (int)g + 1) != (int)g + 1;
             }
@@ -628,7 +631,7 @@ __global__ void __launch_bounds__(TPB) k_r_block(BArgs a, int64_t g) {
     for (int64_t t = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G; t < nh; t += nth) {
         const int q = a.hits[t];
         const int64_t e = a.c_off[q + 1];
-        for (int64_t k = a.head[q] + lg; k < e; k += G) {
+        for (int64_t k = a.cur[q] + lg; k < e; k += G) {
             const int j = a.c_net[k];
             if (active(a, j)) a.status[j] = 2;
         }
@@ -660,6 +663,7 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
                 const int64_t e = a.c_off[q + 1];
                 while (h < e && a.batch[a.c_net[h]] != -1) h++;
                 a.head[q] = h;
+                a.cur[q] = h;
                 const int b = h < e ? a.c_net[h] : -1;
                 a.base[q] = b;
                 a.map[q] = b;
@@ -795,6 +799,7 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
         S.comm.alloc(n);
         S.hits.alloc(XYc);
         S.hstamp.zero(XYc, st);
+        S.cur.alloc(XYc);
     }
     S.stat.zero(3, st);
     S.ctrl.zero(2 * C_WORDS + 1, st);
@@ -802,7 +807,8 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
             N.hs_off.p, N.hs.p, N.vs_off.p, N.vs.p, S.map.p, S.base.p, S.c_off.p, nullptr, S.head.p,
             S.batch.p, S.status.p, S.ctrl.p, S.lists.p,
             release ? S.rel.p : nullptr, release ? S.cand.p : nullptr, release ? S.cstamp.p : nullptr,
-            release ? S.comm.p : nullptr, release ? S.hits.p : nullptr, release ? S.hstamp.p : nullptr, S.stat.p,
+            release ? S.comm.p : nullptr, release ? S.hits.p : nullptr, release ? S.hstamp.p : nullptr,
+            release ? S.cur.p : nullptr, S.stat.p,
             c->g.X, c->g.Y,
             (mode == 1 || mode == 2) ? 2 : 15};
     // Per-cell lists of the nets' point cells, highest priority first.
