@@ -114,3 +114,24 @@ def test_heavy_nets_score_as_the_route_file():
     assert h["opens"] >= 1
     _same(r.evaluate(d, (off, data)), h)
     r.close()
+
+
+def test_planted_oracle_routes():
+    # Oracle-routed small designs with the planted changes of the CPU check of
+    # these rules (dropped lines, stray wires and vias, wires touching end to
+    # end, malformed lines), passed in from the host.
+    import orc
+    from instantgr_amd.host import Design
+    from test_eval_model import _plant
+    for seed in (5, 6):
+        d = Design.custom(layers=6, xsize=30, ysize=28, nets=160, pin_cap=8, sigma=4.0, seed=seed).build(2)
+        o = orc.Oracle.from_design(d)
+        o.set_nets_from_design(d)
+        o.pattern_route(d.array("order"))
+        off, data = o.routes_flat()
+        off2, data2 = _plant(d, off, data)
+        r = gpu.Router.from_design(d)
+        r.set_nets_from_design(d)
+        for o_, d_ in ((off, data), (off2, data2)):
+            _same(r.evaluate(d, (o_, d_)), _host_score(d, o_, d_))
+        r.close()
