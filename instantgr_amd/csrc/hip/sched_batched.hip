@@ -532,7 +532,8 @@ __global__ void __launch_bounds__(TPB) k_r_claim(BArgs a, int64_t g) {
     }
 }
 
-// 2. active nets owning every point cell go to the viable list
+// 2. the nets under test (every unassigned net in round 0, the new owners of
+//    released cells later) that own every point cell go to the viable list
 template <int G>
 __global__ void __launch_bounds__(TPB) k_r_viable(BArgs a, int64_t g, int* vlist) {
     int* st = a.ctrl + (g & 1) * C_WORDS;
