@@ -1,0 +1,53 @@
+"""bench.py's contract logic on the CPU: which kernel the roofline prices, the
+traffic figure it takes from the committed counter summary, and the refusal
+of more than one GPU."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+
+def test_roofline_prices_the_dominant_kernel_with_a_byte_model():
+    # ms, launches per class; the scheduler holds the most time but has no
+    # byte model, so the DAG kernel is priced and the scheduler is named
+    kt = {"scheduler": (500.0, 4), "dag_dp": (400.0, 500), "dag_edges": (180.0, 500), "ripup": (60.0, 500)}
+    kb = {"dag_dp": 4.0e10, "dag_edges": 2.4e11}
+    r = bench.roofline(kt, kb)
+    assert r["kernel"] == "dag_dp" and r["symbol"] == "k_dag_net"
+    assert r["largest_class"] == "scheduler"
+    assert r["achieved"] == pytest.approx(4.0e10 / 0.4 / 1e9)
+    assert r["frac"] == pytest.approx(r["achieved"] / bench.HBM_PEAK_GBS)
+    assert r["bytes_per_launch"] == pytest.approx(4.0e10 / 500)
+    assert r["avg_launch_us"] == pytest.approx(800.0)
+    assert set(r["classes"]) == set(kt)
+    assert "GB/s" in r["classes"]["dag_edges"] and "GB/s" not in r["classes"]["scheduler"]
+
+
+def test_roofline_without_largest_class_when_priced_kernel_leads():
+    r = bench.roofline({"dag_dp": (400.0, 500), "scheduler": (100.0, 4)}, {"dag_dp": 4.0e10})
+    assert r["kernel"] == "dag_dp" and "largest_class" not in r
+
+
+def test_traffic_comes_from_the_committed_counter_summary():
+    path = os.path.join(ROOT, "profiles", "traffic_config1_batched.txt")
+    want = None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic" and f[1] == "k_dag_net":
+            want = float(f[2])
+    assert want and want > 0
+    assert bench._traffic("k_dag_net") == want
+    assert bench._traffic("no_such_kernel") is None
+
+
+def test_more_than_one_gpu_is_refused():
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode == 2
+    assert "one GPU" in p.stderr
+    assert p.stdout == ""
