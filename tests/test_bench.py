@@ -51,3 +51,47 @@ def test_more_than_one_gpu_is_refused():
     assert p.returncode == 2
     assert "one GPU" in p.stderr
     assert p.stdout == ""
+
+
+def _csv(path, header, rows):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        f.write(",".join(header) + "\n")
+        for r in rows:
+            f.write(",".join('"%s"' % v for v in r) + "\n")
+
+
+def test_kernel_names_from_demangled_symbols():
+    assert bench._short("void igr_dev::(anonymous namespace)::k_dag_net<2>(igr_dev::DagArgs, int)") == "k_dag_net"
+    assert bench._short("void igr_dev::(anonymous namespace)::k_r_claim<16>(BArgs, long)", True) == "k_r_claim<16>"
+    assert bench._short("__amd_rocclr_copyBuffer(char*, char*)") == "__amd_rocclr_copyBuffer"
+
+
+def test_counter_traffic_is_scaled_by_the_copy_probe(tmp_path, capsys):
+    # The copy probe reads and writes 1 GiB; FETCH_SIZE reads half of it
+    # (factor 2), WRITE_SIZE all of it (factor 1).
+    hdr = ["Kernel_Name", "Counter_Name", "Counter_Value"]
+    _csv(str(tmp_path / "f" / "a" / "x_counter_collection.csv"), hdr,
+         [("k_copy_probe(int*)", "FETCH_SIZE", 524288), ("k_dag_net<2>(A)", "FETCH_SIZE", 900),
+          ("k_dag_net<2>(A)", "FETCH_SIZE", 1100)])
+    _csv(str(tmp_path / "w" / "a" / "x_counter_collection.csv"), hdr,
+         [("k_copy_probe(int*)", "WRITE_SIZE", 1048576), ("k_dag_net<2>(A)", "WRITE_SIZE", 500),
+          ("k_dag_net<2>(A)", "WRITE_SIZE", 500)])
+    bench.summarize_pmc(str(tmp_path / "f"), str(tmp_path / "w"))
+    out = capsys.readouterr().out
+    assert "factors 2.0000 (fetch), 1.0000 (write)" in out
+    assert "traffic k_dag_net 2560000" in out            # 1000 KiB * 2 + 500 KiB * 1
+    assert "traffic k_copy_probe %d" % (2 << 30) in out
+
+
+def test_lds_conflict_rate(tmp_path, capsys):
+    hdr = ["Kernel_Name", "Counter_Name", "Counter_Value"]
+    _csv(str(tmp_path / "l" / "x_counter_collection.csv"), hdr,
+         [("k_dag_edges_wave(A)", "SQ_LDS_BANK_CONFLICT", 10), ("k_dag_edges_wave(A)", "SQ_LDS_IDX_ACTIVE", 400),
+          ("k_dag_edges_wave(A)", "SQ_LDS_BANK_CONFLICT", 10), ("k_dag_edges_wave(A)", "SQ_LDS_IDX_ACTIVE", 600),
+          ("k_presum<16>(B)", "SQ_LDS_BANK_CONFLICT", 0), ("k_presum<16>(B)", "SQ_LDS_IDX_ACTIVE", 0)])
+    bench.summarize_lds(str(tmp_path / "l"))
+    out = capsys.readouterr().out.splitlines()
+    edges = [l for l in out if l.startswith("k_dag_edges_wave")][0].split()
+    assert edges[1] == "2" and edges[-1] == "0.0200"
+    assert [l for l in out if l.startswith("k_presum")][0].endswith("no LDS")
