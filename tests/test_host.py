@@ -6,6 +6,7 @@ import hashlib
 import itertools
 import os
 import re
+import shutil
 import subprocess
 
 import numpy as np
@@ -279,6 +280,16 @@ def test_abi_exports_every_declared_symbol():
         exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
         missing = sorted(n for n in names if n not in exported)
         assert not missing, (header, missing)
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")
+def test_abi_headers_are_plain_c():
+    # The C boundary: both headers compile as C99 with no warnings, so a cgo,
+    # JNI or ctypes binding can include them unchanged.
+    for header in ("igr.h", "igr_host.h"):
+        p = subprocess.run(["gcc", "-fsyntax-only", "-Wall", "-Wextra", "-Werror", "-std=c99", "-pedantic", "-x", "c",
+                            os.path.join(ROOT, "include", header)], capture_output=True, text=True)
+        assert p.returncode == 0, (header, p.stderr)
 
 
 def test_cli_argument_errors(tmp_path):
