@@ -8,8 +8,14 @@ checked where the mathematics fixes its answer:
     packed keys);
   * a two-node DAG with a long straight edge is brute-forced over the end
     layers, the switch layers and the sampled switch points.
-Multi-parent DAGs (detour alternatives) have no closed form: their routes are
-pinned only by the agreement of the HIP kernels with this oracle."""
+  * a multi-parent DAG whose alternatives share a slot of their parent (both
+    L corners of every diagonal tree edge, the child a child of each corner)
+    is routed at the cost of its best selection tree: the minimum over every
+    choice of one corner per diagonal edge, each routed as a tree.  In all
+    ten cases the choice changes the cost, and in six the best is not the
+    first corner.
+Mirror trunks with duplicated pin nodes use the same per-slot minimum; they
+are pinned only by the agreement of the HIP kernels with this oracle."""
 import math
 
 import numpy as np
@@ -184,3 +190,107 @@ def test_dag_two_node_layer_change_brute_force(seed):
     orc.dag_pass(o, np.array([0], np.int32), _tree_dag(nodes, parents, X, Y))
     got = _route_cost(o.route(0), L, X, Y, ps, vc)
     assert math.isclose(got, brute, rel_tol=1e-6), (got, brute)
+
+
+def _diag_tree(rng, L, X, Y, n):
+    """Random tree of n nodes at distinct gcells, parents before children,
+    with at least one diagonal edge (both coordinates differ)."""
+    XY = X * Y
+    while True:
+        pos, parents = [], []
+        while len(pos) < n:
+            p = (int(rng.integers(0, X)), int(rng.integers(0, Y)))
+            if p not in pos:
+                parents.append(-1 if not pos else int(rng.integers(0, len(pos))))
+                pos.append(p)
+        if any(pos[i][0] != pos[parents[i]][0] and pos[i][1] != pos[parents[i]][1] for i in range(1, n)):
+            break
+    layers = [int(rng.choice([0, 1, L])) for _ in range(n)]
+    layers[0] = int(rng.choice([0, 1]))
+    return [layers[i] * XY + pos[i][0] * Y + pos[i][1] for i in range(n)], [-1] + parents[1:], pos
+
+
+def _lcorner_dag(nodes, parents, pos, X, Y, L, pick=None):
+    """DAG of a tree whose diagonal edges get both L corners as alternatives,
+    as the detour builder makes them: Steiner nodes at (parent x, child y) and
+    (child x, parent y), each in the parent's slot of that child and each a
+    parent of the child through slot 0, so the child has two parents.  With
+    pick, diagonal edge e keeps only corner pick[e] (a selection tree)."""
+    XY, n = X * Y, len(nodes)
+    kids = [[] for _ in range(n)]
+    for i in range(1, n):
+        kids[parents[i]].append(i)
+    out_nodes, out_par, child_num = [], [], []     # out_par[i]: list of (parent, slot)
+    diag = 0
+
+    def emit(v, par_list):
+        out_nodes.append(v)
+        out_par.append(par_list)
+        child_num.append(0)
+        return len(out_nodes) - 1
+
+    def visit(v, par_list):
+        nonlocal diag
+        me = emit(nodes[v], par_list)
+        child_num[me] = len(kids[v])
+        for s, c in enumerate(kids[v]):
+            (px, py), (cx, cy) = pos[v], pos[c]
+            if px != cx and py != cy:
+                corners = [(px, cy), (cx, py)]
+                keep = [0, 1] if pick is None else [pick[diag]]
+                diag += 1
+                tl = []
+                for k in keep:
+                    t = emit(L * XY + corners[k][0] * Y + corners[k][1], [(me, s)])
+                    child_num[t] = 1
+                    tl.append(t)
+                visit(c, [(t, 0) for t in tl])
+            else:
+                visit(c, [(me, s)])
+    visit(0, [])
+    depth = [0] * len(out_nodes)
+    for i in range(1, len(out_nodes)):
+        depth[i] = max(depth[p] + 1 for p, _ in out_par[i])
+    par_off, par, slot = [0], [], []
+    for pl in out_par:
+        for p, s in pl:
+            par.append(p)
+            slot.append(s)
+        par_off.append(len(par))
+    pts = sorted({v % XY for v in out_nodes})
+    return {"dag_node_off": np.array([0, len(out_nodes)], np.int64), "dag_node": np.array(out_nodes, np.int32),
+            "dag_par_off": np.array(par_off, np.int64), "dag_par": np.array(par, np.int32),
+            "dag_slot": np.array(slot, np.int32), "dag_child_num": np.array(child_num, np.int32),
+            "dag_depth": np.array(depth, np.int32), "dag_pt_off": np.array([0, len(pts)], np.int64),
+            "dag_pt": np.array(pts, np.int32)}, diag
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_multi_parent_dag_costs_the_best_selection_tree(seed):
+    # A node with several parents (one per alternative in the same slot of
+    # its grandparent) is routed through the best alternative: the DAG's
+    # route costs the minimum over every choice of one corner per diagonal
+    # edge, each of those trees routed by the tree DP (pinned above).
+    import itertools
+    rng = np.random.default_rng(900 + seed)
+    L, X, Y, DIR = 4 + seed % 2, 12, 11, seed % 2
+    dem = (rng.integers(0, 12, L * X * Y) * 0.5).astype(np.float32)
+    _, vc, ps = _costs(L, X, Y, DIR, seed, dem)
+    nodes, parents, pos = _diag_tree(rng, L, X, Y, 2 + seed % 3)
+    tree = (np.array([0, len(nodes)], np.int64), np.array(nodes, np.int32), np.array(parents, np.int32))
+
+    def routed_cost(pick):
+        o, _, _ = _costs(L, X, Y, DIR, seed, dem)
+        o.set_nets(*tree)
+        dag, _ = _lcorner_dag(nodes, parents, pos, X, Y, L, pick)
+        assert orc.dag_pass(o, np.array([0], np.int32), dag) == 1
+        return _route_cost(o.route(0), L, X, Y, ps, vc)
+    _, k = _lcorner_dag(nodes, parents, pos, X, Y, L)
+    assert dag_has_two_parents(_lcorner_dag(nodes, parents, pos, X, Y, L)[0])
+    sel = [routed_cost(list(p)) for p in itertools.product([0, 1], repeat=k)]
+    got = routed_cost(None)
+    assert math.isclose(got, min(sel), rel_tol=1e-6), (got, sel)
+
+
+def dag_has_two_parents(dag):
+    return bool(np.any(np.diff(dag["dag_par_off"]) >= 2))
