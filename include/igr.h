@@ -7,7 +7,9 @@
  * copied (the caller keeps ownership), outputs go to caller-allocated host
  * buffers, every call returns IGR_OK (0) or a negative status and never exits
  * the process, igr_last_error() describes the last failure of the calling
- * thread, and one host thread drives a context at a time.
+ * thread, and one host thread drives a context at a time.  Net lists
+ * (order, nets) name each uploaded net at most once; a list with a repeated
+ * or out-of-range id is rejected with IGR_EINVAL before any device work.
  *
  * Grid arrays use the reference cell index l*X*Y + x*Y + y over routing layers
  * (metal layer - 1), see src/database_cuda.hpp:49 and :2106.

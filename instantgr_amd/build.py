@@ -37,7 +37,7 @@ def _newer(out, deps):
     return any(os.path.getmtime(d) > t for d in deps)
 
 
-def _objs(srcs, subdir, compiler, flags, ext_out=".o"):
+def _objs(srcs, subdir, compiler, flags, force=False, ext_out=".o"):
     objdir = os.path.join(LIB, "obj", subdir)
     os.makedirs(objdir, exist_ok=True)
     headers = []
@@ -49,7 +49,7 @@ def _objs(srcs, subdir, compiler, flags, ext_out=".o"):
         src = os.path.join(CSRC, subdir, s)
         out = os.path.join(objdir, s + ext_out)
         outs.append(out)
-        if _newer(out, [src] + headers):
+        if force or _newer(out, [src] + headers):
             jobs.append([compiler] + flags + ["-c", src, "-o", out])
     procs = []
     for j in jobs:
@@ -68,7 +68,7 @@ def build_host(force=False):
     os.makedirs(LIB, exist_ok=True)
     flags = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-result",
              "-I" + INCLUDE, "-I" + os.path.join(CSRC, "host")]
-    objs = _objs(HOST_SRCS, "host", "g++", flags)
+    objs = _objs(HOST_SRCS, "host", "g++", flags, force)
     out = os.path.join(LIB, "libigr_host.so")
     if force or _newer(out, objs):
         _run(["g++", "-shared", "-o", out] + objs + ["-lpthread"])
@@ -82,7 +82,7 @@ def build_hip(force=False):
              "-munsafe-fp-atomics", "-Wno-unused-result",
              "-I" + INCLUDE, "-I" + os.path.join(CSRC, "host"), "-I" + os.path.join(CSRC, "hip")]
     srcs = [s for s in HIP_SRCS if os.path.exists(os.path.join(CSRC, "hip", s))]
-    objs = _objs(srcs, "hip", HIPCC, flags)
+    objs = _objs(srcs, "hip", HIPCC, flags, force)
     out = os.path.join(LIB, "libigr.so")
     if force or _newer(out, objs + [host]):
         _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", out] + objs +

@@ -20,6 +20,11 @@ static thread_local std::string g_err;
 
 const char* igr_last_error(void) { return g_err.c_str(); }
 
+int igr_dev::fail(int status, const std::string& msg) {
+    g_err = msg;
+    return status;
+}
+
 template <typename F>
 static int guard(F&& f) {
     try {
@@ -245,12 +250,24 @@ int igr_set_nets(igr_ctx* c, const igr_net_desc* d) {
     });
 }
 
+namespace {
+// Net lists name each net at most once: a repeated net would be ripped up
+// twice in one pass and its route slot released while still in use.
+void check_nets(const igr_ctx* c, const int* nets, int64_t n) {
+    if (n < 0 || (n > 0 && !nets)) throw std::invalid_argument("bad net list");
+    std::vector<uint8_t> seen((size_t)c->nets.n, 0);
+    for (int64_t i = 0; i < n; i++) {
+        if (nets[i] < 0 || nets[i] >= c->nets.n) throw std::invalid_argument("net id out of range");
+        if (seen[nets[i]]) throw std::invalid_argument("net listed twice");
+        seen[nets[i]] = 1;
+    }
+}
+} // namespace
+
 int igr_schedule(igr_ctx* c, const int* order, int64_t n, int mode, int form, int* batch_of, int* nbatches) {
     return guard([&]() {
         if (form != IGR_FORM_PLAIN && form != IGR_FORM_BATCHED) throw std::invalid_argument("unknown form");
-        if (n > c->nets.n) throw std::invalid_argument("more nets than uploaded");
-        for (int64_t i = 0; i < n; i++)
-            if (order[i] < 0 || order[i] >= c->nets.n) throw std::invalid_argument("net id out of range");
+        check_nets(c, order, n);
         *nbatches = form == IGR_FORM_PLAIN ? run_schedule(c, order, n, mode, batch_of)
                                            : run_schedule_batched(c, order, n, mode, batch_of);
     });
@@ -285,10 +302,6 @@ void plain_batch(igr_ctx* c, const int* d_nets, const int64_t* d_off, int64_t n,
     launch_via_commit(c, d_nets, n, stamp, 1);
 }
 
-void check_nets(const igr_ctx* c, const int* nets, int64_t n) {
-    for (int64_t i = 0; i < n; i++)
-        if (nets[i] < 0 || nets[i] >= c->nets.n) throw std::invalid_argument("net id out of range");
-}
 
 // Pattern routing writes the pattern slot: point nets that a rip-up pass
 // moved elsewhere back at it.

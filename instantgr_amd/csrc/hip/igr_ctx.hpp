@@ -266,6 +266,9 @@ struct KScope {
     }
 };
 
+// Records msg as the calling thread's igr_last_error() and returns status.
+int fail(int status, const std::string& msg);
+
 // IGR_PROGRESS set in the environment: one line per stage on stderr, for
 // long runs watched from outside.
 void progress(const char* fmt, ...);

@@ -29,8 +29,11 @@ double now_s() {
 int igr_run(igr_ctx* c, igrh_design* h, const igr_run_opts* o, igr_run_stats* st) {
     igr_run_stats S{};
     const int form = o ? o->form : IGR_FORM_PLAIN;
-    const int iters = o ? o->rrr_iterations : 3;
+    const int iters = o ? o->rrr_iterations : 1;
     const int threads = o && o->threads > 0 ? o->threads : 8;
+    if (!c || !h) return fail(IGR_EINVAL, "null context or design");
+    if (form != IGR_FORM_PLAIN && form != IGR_FORM_BATCHED) return fail(IGR_EINVAL, "unknown form");
+    if (iters < 0 || iters > IGR_MAX_RRR) return fail(IGR_EINVAL, "rip-up-and-reroute passes outside 0..IGR_MAX_RRR");
     double t0 = now_s();
     const igr::PatternInputs& pi = *igrh_inputs_ref(h);
     const igr::Subnets& sn = *igrh_subnets_ref(h);
@@ -52,7 +55,7 @@ int igr_run(igr_ctx* c, igrh_design* h, const igr_run_opts* o, igr_run_stats* st
     const int64_t XYc = (int64_t)c->g.X * c->g.Y;
     std::vector<float> xs(XYc), ys(XYc);
     std::vector<uint8_t> is_of(pi.n);
-    for (int it = 0; it < iters && it < IGR_MAX_RRR; it++) {
+    for (int it = 0; it < iters; it++) {
         double a = now_s();
         if ((rc = igr_congestion_view(c, xs.data(), ys.data()))) return rc;
         if ((rc = igr_ripup_mark(c, 0, is_of.data()))) return rc;
@@ -64,7 +67,7 @@ int igr_run(igr_ctx* c, igrh_design* h, const igr_run_opts* o, igr_run_stats* st
         if (nets.empty()) break;
         double b = now_s();
         if ((rc = igrh_build_dags(h, nets.data(), (int64_t)nets.size(), xs.data(), ys.data(), mode, threads))) {
-            return IGR_EINVAL;
+            return fail(IGR_EINVAL, std::string("detour DAGs: ") + igrh_error());
         }
         double cc = now_s();
         const igr::DagSet& D = *igrh_dags_ref(h);

@@ -115,6 +115,15 @@ int evaluate_routes(const Design& d, const std::string& out_path, Score& sc) {
                 std::fclose(f);
                 throw std::runtime_error("malformed route line");
             }
+            // A line leaving the grid opens its net and touches no cell (the
+            // device evaluator's rule for cells outside the grid).  A via
+            // written top-down adds nothing, as in the reference's unsigned
+            // layer loop (run/evaluator.cpp:525-530).
+            if (xl < 0 || yl < 0 || zl < 0 || xh < 0 || yh < 0 || zh < 0 || xl >= X || xh >= X || yl >= Y ||
+                yh >= Y || zl >= L || zh >= L) {
+                bad = true;
+                continue;
+            }
             if (zh != zl) {
                 if (xh == xl && yh == yl) {
                     for (int z = zl; z < zh; z++) { vias[z]++; via_loc.push_back(V{xl, yl, z}); }

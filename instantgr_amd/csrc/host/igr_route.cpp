@@ -3,13 +3,15 @@
 //   igr_route -cap <file> -net <file> -out <file> [-form plain|batched]
 //             [-rrr N] [-threads T]
 // Reads the .cap/.net files, builds the Steiner trees, routes on the GPU
-// (pattern pass plus N rip-up-and-reroute passes, default 3) and writes the
-// route file in the reference format.  Exits non-zero on any error.
+// (pattern pass plus N rip-up-and-reroute passes; the default 1 is what the
+// reference's route() runs, src/main.cpp:33-71) and writes the route file in
+// the reference format.  Exits non-zero on any error, 2 on a bad command line.
 #include "igr.h"
 #include "igr_host.h"
 
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -20,22 +22,42 @@ static double now_s() {
 
 int main(int argc, char** argv) {
     std::string cap, net, out;
-    igr_run_opts opts{IGR_FORM_BATCHED, 3, 8};
+    igr_run_opts opts{IGR_FORM_BATCHED, 1, 8};
+    auto number = [](const std::string& v, int lo, int hi, int* out) {
+        char* end = nullptr;
+        const long x = std::strtol(v.c_str(), &end, 10);
+        if (v.empty() || *end || x < lo || x > hi) return false;
+        *out = (int)x;
+        return true;
+    };
+    const char* usage = "usage: %s -cap <file> -net <file> -out <file> [-form plain|batched] [-rrr 0..%d] "
+                        "[-threads 1..256]\n";
+    if (argc % 2 != 1) {
+        std::fprintf(stderr, usage, argv[0], IGR_MAX_RRR);
+        return 2;
+    }
     for (int i = 1; i + 1 < argc; i += 2) {
         std::string k = argv[i], v = argv[i + 1];
+        bool ok = true;
         if (k == "-cap") cap = v;
         else if (k == "-net") net = v;
         else if (k == "-out") out = v;
-        else if (k == "-form") opts.form = v == "plain" ? IGR_FORM_PLAIN : IGR_FORM_BATCHED;
-        else if (k == "-rrr") opts.rrr_iterations = std::stoi(v);
-        else if (k == "-threads") opts.threads = std::stoi(v);
+        else if (k == "-form") {
+            ok = v == "plain" || v == "batched";
+            opts.form = v == "plain" ? IGR_FORM_PLAIN : IGR_FORM_BATCHED;
+        } else if (k == "-rrr") ok = number(v, 0, IGR_MAX_RRR, &opts.rrr_iterations);
+        else if (k == "-threads") ok = number(v, 1, 256, &opts.threads);
         else {
             std::fprintf(stderr, "unknown option %s\n", k.c_str());
             return 2;
         }
+        if (!ok) {
+            std::fprintf(stderr, "bad value '%s' for %s\n", v.c_str(), k.c_str());
+            return 2;
+        }
     }
     if (cap.empty() || net.empty() || out.empty()) {
-        std::fprintf(stderr, "usage: %s -cap <file> -net <file> -out <file> [-form plain|batched] [-rrr N]\n", argv[0]);
+        std::fprintf(stderr, usage, argv[0], IGR_MAX_RRR);
         return 2;
     }
     double t0 = now_s();

@@ -44,11 +44,15 @@ struct Scanner {
         while (p < end && !std::isdigit((unsigned char)*p)) ++p;
         if (p >= end) fail("unexpected end of file");
     }
-    long long read_int() {
+    // Unsigned decimal field; values above INT_MAX are malformed input.
+    int read_int() {
         skip_to_digit();
         long long v = 0;
-        while (p < end && std::isdigit((unsigned char)*p)) v = v * 10 + (*p++ - '0');
-        return v;
+        while (p < end && std::isdigit((unsigned char)*p)) {
+            v = v * 10 + (*p++ - '0');
+            if (v > 0x7fffffffLL) fail("integer field out of range");
+        }
+        return (int)v;
     }
     // Digits with an optional '.', as the reference scanner reads them.
     double read_double() {
@@ -75,9 +79,9 @@ struct Scanner {
 void read_cap(const std::string& path, Design& d) {
     std::vector<char> buf = slurp(path);
     Scanner s{buf.data(), buf.data() + buf.size() - 1, "cap file '" + path + "'"};
-    d.L = (int)s.read_int();
-    d.X = (int)s.read_int();
-    d.Y = (int)s.read_int();
+    d.L = s.read_int();
+    d.X = s.read_int();
+    d.Y = s.read_int();
     if (d.L < 2 || d.X < 2 || d.Y < 2) s.fail("grid must have at least 2 layers and 2x2 gcells");
     if ((long long)d.L * d.X * d.Y > 0x7fffffffLL) s.fail("grid too large for 32-bit cell indices");
     d.wire_cost = s.read_double();
@@ -86,15 +90,15 @@ void read_cap(const std::string& path, Design& d) {
     for (int l = 0; l < d.L; l++) d.of_weight[l] = s.read_double();
     d.xlen.resize(d.X - 1);
     d.ylen.resize(d.Y - 1);
-    for (int i = 0; i < d.X - 1; i++) d.xlen[i] = (int)s.read_int();
-    for (int i = 0; i < d.Y - 1; i++) d.ylen[i] = (int)s.read_int();
+    for (int i = 0; i < d.X - 1; i++) d.xlen[i] = s.read_int();
+    for (int i = 0; i < d.Y - 1; i++) d.ylen[i] = s.read_int();
     d.dir.resize(d.L);
     d.min_len.resize(d.L);
     d.cap.assign((size_t)d.L * d.X * d.Y, 0.0);
     for (int l = 0; l < d.L; l++) {
         std::string name = s.read_token();
         if (name != "metal" + std::to_string(l + 1)) s.fail(("unexpected layer name " + name).c_str());
-        d.dir[l] = (int)s.read_int();
+        d.dir[l] = s.read_int();
         if (d.dir[l] != 0 && d.dir[l] != 1) s.fail("layer direction must be 0 or 1");
         d.min_len[l] = s.read_double();
         for (int y = 0; y < d.Y; y++)
@@ -123,8 +127,9 @@ void read_net(const std::string& path, Design& d) {
         int minx = d.X, maxx = 0, miny = d.Y, maxy = 0;
         for (;;) {                       // pins
             for (;;) {                   // access points of one pin
-                int l = (int)s.read_int(), x = (int)s.read_int(), y = (int)s.read_int();
-                if (l >= d.L || x >= d.X || y >= d.Y) s.fail("access point outside the grid");
+                int l = s.read_int(), x = s.read_int(), y = s.read_int();
+                if (l < 0 || x < 0 || y < 0 || l >= d.L || x >= d.X || y >= d.Y)
+                    s.fail("access point outside the grid");
                 minx = std::min(minx, x); maxx = std::max(maxx, x);
                 miny = std::min(miny, y); maxy = std::max(maxy, y);
                 d.ap.push_back(l * XY + x * d.Y + y);

@@ -221,6 +221,8 @@ class Router:
 
     def set_demand(self, d):
         d = np.ascontiguousarray(d, np.float32)
+        if d.size != self.n:
+            raise ValueError("demand needs L*X*Y = %d values, got %d" % (self.n, d.size))
         _check(lib().igr_set_demand(self.h, d.ctypes.data))
 
     def demand(self):
@@ -284,10 +286,20 @@ class Router:
         """One rip-up-and-reroute pass over the DAGs returned by
         instantgr_amd.host.Design.build_dags for the same net list."""
         nets = np.ascontiguousarray(nets, np.int32)
-        keys = ("dag_node_off", "dag_node", "dag_par_off", "dag_par", "dag_slot", "dag_child_num", "dag_depth",
-                "dag_pt_off", "dag_pt")
-        arrs = [np.ascontiguousarray(dags[k]) for k in keys]
-        d = DagDesc(len(nets), *[a.ctypes.data for a in arrs])
+        keys = (("dag_node_off", np.int64), ("dag_node", np.int32), ("dag_par_off", np.int64),
+                ("dag_par", np.int32), ("dag_slot", np.int32), ("dag_child_num", np.int32),
+                ("dag_depth", np.int32), ("dag_pt_off", np.int64), ("dag_pt", np.int32))
+        arrs = [np.ascontiguousarray(dags[k], dt) for k, dt in keys]
+        node_off, node, par_off, par, slot, cnum, depth, pt_off, pt = arrs
+        n = len(nets)
+        if len(node_off) != n + 1 or len(pt_off) != n + 1:
+            raise ValueError("dag_node_off and dag_pt_off need one entry per net plus one")
+        nodes = int(node_off[-1])
+        if len(node) < nodes or len(cnum) < nodes or len(depth) < nodes or len(par_off) != nodes + 1:
+            raise ValueError("DAG node arrays shorter than dag_node_off says")
+        if len(par) < par_off[-1] or len(slot) < par_off[-1] or len(pt) < pt_off[-1]:
+            raise ValueError("DAG edge or point arrays shorter than their offsets say")
+        d = DagDesc(n, *[a.ctypes.data for a in arrs])
         st = PassStats()
         _check(lib().igr_dag_pass(self.h, nets.ctypes.data, len(nets), mode, form, C.byref(d), C.byref(st)))
         return {"batches": st.batches, "schedule_s": st.schedule_s, "route_s": st.route_s, "nets": st.nets}

@@ -535,6 +535,49 @@ int orc_pattern_route(orc_ctx* c, const int* order, int64_t n) {
     return nb;
 }
 
+// Grid state and routes taken over from elsewhere (a routed device context),
+// so that a later pass can be replayed from the same state.
+int orc_set_state(orc_ctx* c, const float* demand, const int* ts, int stamp, const int64_t* route_off,
+                  const int* routes) {
+    std::memcpy(c->demand.data(), demand, sizeof(float) * c->N());
+    std::memcpy(c->timestamp.data(), ts, sizeof(int) * c->N());
+    c->stamp = stamp;
+    if (route_off) {
+        for (size_t i = 0; i < c->routes.size(); i++) {
+            const int* r = routes + route_off[i];
+            if (r[0] < 1 || r[0] > route_off[i + 1] - route_off[i]) return -1;
+            c->routes[i].assign(r, r + r[0]);
+        }
+    }
+    return 0;
+}
+
+// One batch of given routes committed as the reference commits a routed
+// batch (src/Lshape_route.hpp:240-241: wire differences, batch_wire_update,
+// then commit_via_demand), or ripped up as the DAG pass does
+// (src/Lshape_route_detour.hpp:1047-1048: commit_wire_demand, then
+// commit_via_demand, both with K = -1).  Each route is buf[0] = next free
+// index, then cell pairs; wires run along one layer between the pair's cells.
+void orc_commit_routes(orc_ctx* c, int64_t n, const int64_t* route_off, const int* routes, int K) {
+    const int stamp = ++c->stamp;
+    std::vector<std::vector<int>> rs(n);
+    for (int64_t i = 0; i < n; i++) rs[i].assign(routes + route_off[i], routes + route_off[i] + routes[route_off[i]]);
+    if (K < 0) {
+        for (auto& r : rs) commit_wire_demand(c, r, stamp, -1);
+        for (auto& r : rs) commit_via_demand(c, r, stamp, -1);
+        return;
+    }
+    for (auto& r : rs)
+        for (size_t k = 1; k + 1 < (size_t)r[0]; k += 2) {
+            const int a = std::min(r[k], r[k + 1]), b = std::max(r[k], r[k + 1]);
+            const int la = a / c->X / c->Y, lb = b / c->X / c->Y;
+            if (la != lb) continue;
+            add_wire_segment(c, la, a / c->Y % c->X, b / c->Y % c->X, a % c->Y, b % c->Y, stamp);
+        }
+    batch_wire_update(c, stamp);
+    for (auto& r : rs) commit_via_demand(c, r, stamp, 1);
+}
+
 int orc_get_route(const orc_ctx* c, int net, int* buf, int cap) {
     const auto& r = c->routes[net];
     int n = std::min((int)r.size(), cap);

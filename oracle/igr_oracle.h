@@ -61,6 +61,17 @@ void orc_lshape_batch(orc_ctx* c, const int* nets, int64_t n);
 int orc_pattern_route(orc_ctx* c, const int* order, int64_t n);
 
 int orc_get_route(const orc_ctx* c, int net, int* buf, int cap);
+
+/* Replaces demand, timestamps, the stamp counter and (route_off non-NULL)
+ * every net's route (route_off[i] indexes routes; routes[route_off[i]] is the
+ * route's next free index).  Returns -1 on a route longer than its slot. */
+int orc_set_state(orc_ctx* c, const float* demand, const int* ts, int stamp, const int64_t* route_off,
+                  const int* routes);
+/* Commits n given routes as one batch (K = 1: wire differences,
+ * batch_wire_update, via commit; src/Lshape_route.hpp:240-241) or rips them
+ * up (K = -1: commit_wire_demand then commit_via_demand;
+ * src/Lshape_route_detour.hpp:1047-1048), with a new stamp. */
+void orc_commit_routes(orc_ctx* c, int64_t n, const int64_t* route_off, const int* routes, int K);
 int orc_stamp(const orc_ctx* c);
 
 /* Rip-up marking (src/graph.hpp:432-466,588-601): is_of[net] = 1 when more

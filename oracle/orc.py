@@ -172,6 +172,35 @@ class Oracle:
         lib().orc_congestion_view(self.h, xs.ctypes.data, ys.ctypes.data)
         return xs, ys
 
+    def set_state(self, demand, ts, stamp, routes=None):
+        """Take over demand, timestamps, the stamp counter and (routes =
+        (off, data) as gpu.Router.routes_flat gives) every net's route."""
+        L = lib()
+        L.orc_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        L.orc_set_state.restype = C.c_int
+        dem = np.ascontiguousarray(demand, np.float32)
+        t = np.ascontiguousarray(ts, np.int32)
+        assert dem.size == self.n and t.size == self.n
+        off = data = None
+        if routes is not None:
+            off = np.ascontiguousarray(routes[0], np.int64)
+            data = np.ascontiguousarray(routes[1], np.int32)
+            assert len(off) == self.num_nets + 1
+        if L.orc_set_state(self.h, dem.ctypes.data, t.ctypes.data, int(stamp), _p(off), _p(data)) != 0:
+            raise RuntimeError("route longer than its slot")
+
+    def commit_routes(self, routes, K=1):
+        """Commit (K=1) or rip up (K=-1) a list of routes as one batch."""
+        L = lib()
+        L.orc_commit_routes.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int]
+        rs = [np.ascontiguousarray(r, np.int32) for r in routes]
+        off = np.zeros(len(rs) + 1, np.int64)
+        off[1:] = np.cumsum([len(r) for r in rs])
+        data = np.concatenate(rs) if rs else np.zeros(1, np.int32)
+        for r in rs:
+            assert r[0] == len(r)
+        L.orc_commit_routes(self.h, len(rs), off.ctypes.data, data.ctypes.data, int(K))
+
     def lshape_probe(self, net):
         buf = np.empty(8192, np.int32)
         best = lib().orc_lshape_probe(self.h, net, buf.ctypes.data, buf.size)

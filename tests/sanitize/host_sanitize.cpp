@@ -6,6 +6,7 @@
 #include "igr_host.h"
 
 #include <cstdio>
+#include <cstring>
 #include <random>
 #include <string>
 #include <vector>
@@ -52,6 +53,32 @@ int main(int argc, char** argv) {
         }
         igrh_evaluate(h, out.c_str(), &sc);
         printf("preset %d: %lld subnets, %lld open nets\n", preset, (long long)gi.subnets, (long long)sc.opens);
+        // Malformed lines inside the first net: outside the grid on every
+        // axis, negative, a via past the top layer, a via written top-down.
+        {
+            std::string text;
+            if (FILE* f = std::fopen(out.c_str(), "rb")) {
+                char buf[1 << 16];
+                size_t k;
+                while ((k = std::fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, k);
+                std::fclose(f);
+            }
+            const size_t at = text.find("(\n");
+            if (at == std::string::npos) { printf("route file without a net block\n"); return 1; }
+            char bad[512];
+            std::snprintf(bad, sizeof bad,
+                          "%d 0 1 %d 0 1\n0 %d 2 0 %d 2\n-1 0 1 3 0 1\n0 0 -2 0 0 1\n0 0 1 0 0 %d\n3 3 4 3 3 2\n"
+                          "0 0 %d 0 0 %d\n",
+                          X, X + 5, Y, Y + 3, gi.metal_layers + 2, gi.metal_layers, gi.metal_layers + 1);
+            text.insert(at + 2, bad);
+            const std::string out2 = dir + "/s_bad.out";
+            FILE* f = std::fopen(out2.c_str(), "wb");
+            std::fwrite(text.data(), 1, text.size(), f);
+            std::fclose(f);
+            igrh_score sb;
+            igrh_evaluate(h, out2.c_str(), &sb);
+            if (sb.opens < 1) { printf("malformed lines did not open the net\n"); return 1; }
+        }
         igrh_free(h);
     }
     printf("clean\n");
