@@ -78,6 +78,16 @@ int64_t igrh_array(const igrh_design* h, const char* name, const void** ptr);
 int igrh_build_dags(igrh_design* h, const int* nets, int64_t n, const float* xsum, const float* ysum, int mode,
                     int threads);
 
+/* Rectilinear Steiner tree of n distinct gcells (x[i], y[i]) on a grid of
+ * xsize*ysize gcells with edge lengths xlen (xsize-1 entries) and ylen
+ * (ysize-1): the tree the subnet builder gives a net with these pins, in
+ * place of flute::flute (src/flute.hpp:1239-1350) as my_flute calls it
+ * (src/database_cuda.hpp:696-744).  Writes the nodes as (x, y) pairs into
+ * nodes_xy (room for node_cap nodes; a tree of n pins has fewer than 2n) and
+ * the *n_nodes - 1 edges as node index pairs into edges. */
+int igrh_steiner(int n, const int* x, const int* y, int xsize, int ysize, const int* xlen, const int* ylen,
+                 int node_cap, int* nodes_xy, int* edges, int* n_nodes);
+
 /* Routes per subnet: off[s] indexes data; data[off[s]] = next free slot,
  * then cell index pairs (routing layers). */
 int igrh_write_routes(const igrh_design* h, const char* out_path, const int64_t* off, const int* data,

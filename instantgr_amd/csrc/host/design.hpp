@@ -95,6 +95,11 @@ void build_grid(const Design& d, Grid& g);
 
 // Subnets and Steiner trees (src/database_cuda.hpp:2109-2256, :696-820).
 void build_subnets(const Design& d, const NetPins& np, const Grid& g, Subnets& s, int threads);
+// Steiner tree of distinct gcells (x[i], y[i]) with the given gcell edge
+// lengths, as the subnet builder makes it: node positions (x, y) and edges.
+void steiner_of_points(const std::vector<int>& x, const std::vector<int>& y, const std::vector<int>& xlen,
+                       const std::vector<int>& ylen, std::vector<std::pair<int, int>>& nodes,
+                       std::vector<std::pair<int, int>>& edges);
 
 // Synthetic design generator (deterministic, own PRNG).
 struct GenConfig {

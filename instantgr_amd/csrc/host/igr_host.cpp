@@ -166,3 +166,30 @@ const igr::Grid* igrh_grid_ref(const igrh_design* h) { return &h->g; }
 const igr::Subnets* igrh_subnets_ref(const igrh_design* h) { return &h->s; }
 const igr::PatternInputs* igrh_inputs_ref(const igrh_design* h) { return &h->pi; }
 const igr::DagSet* igrh_dags_ref(const igrh_design* h) { return &h->dags; }
+
+int igrh_steiner(int n, const int* x, const int* y, int xsize, int ysize, const int* xlen, const int* ylen,
+                 int node_cap, int* nodes_xy, int* edges, int* n_nodes) {
+    return guard([&]() {
+        if (n < 1 || !x || !y || !nodes_xy || !edges || !n_nodes || xsize < 1 || ysize < 1 ||
+            (xsize > 1 && !xlen) || (ysize > 1 && !ylen))
+            throw std::invalid_argument("igrh_steiner: bad arguments");
+        std::vector<int> vx(x, x + n), vy(y, y + n);
+        std::vector<int> lx(xlen, xlen + (xsize - 1)), ly(ylen, ylen + (ysize - 1));
+        for (int v : lx)
+            if (v < 0) throw std::invalid_argument("negative gcell length");
+        for (int v : ly)
+            if (v < 0) throw std::invalid_argument("negative gcell length");
+        std::vector<std::pair<int, int>> nodes, es;
+        steiner_of_points(vx, vy, lx, ly, nodes, es);
+        if ((int)nodes.size() > node_cap) throw std::invalid_argument("igrh_steiner: node_cap too small");
+        for (size_t i = 0; i < nodes.size(); i++) {
+            nodes_xy[2 * i] = nodes[i].first;
+            nodes_xy[2 * i + 1] = nodes[i].second;
+        }
+        for (size_t i = 0; i < es.size(); i++) {
+            edges[2 * i] = es[i].first;
+            edges[2 * i + 1] = es[i].second;
+        }
+        *n_nodes = (int)nodes.size();
+    });
+}

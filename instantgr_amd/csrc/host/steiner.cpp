@@ -224,17 +224,198 @@ void rsmt4(const Coords& C, std::vector<int>& nx, std::vector<int>& ny) {
     if (s2 >= 0) { nx.push_back(cx[s2]); ny.push_back(cy[s2]); }
 }
 
+// Exact rectilinear Steiner minimal tree of 5..9 terminals by the
+// Dreyfus-Wagner recurrence (Dreyfus and Wagner, Networks 1971) over the
+// Hanan grid of the terminals (Hanan 1966: some minimal tree uses only grid
+// points), with L1 distances in gcell-length coordinates.  The last terminal
+// is the root; for every subset D of the others and grid point v,
+//   S(D, v) = min_u dist(v, u) + g(D, u),
+//   g(D, u) = min over splits D = E + F (E holding D's lowest terminal) of
+//             S(E, u) + S(F, u),
+// with S({t}, v) = dist(v, t).  The min over u is an L1 distance transform
+// on the grid (a forward and a backward sweep per axis), so a subset costs
+// O(splits * grid) instead of O(grid^2).  Ties keep the first candidate in
+// sweep and split order, so the tree is deterministic.  The tree is read back
+// from the argmins as edges between grid points; Steiner points of degree
+// two on a shortest path between their neighbours are dropped.  Returns the
+// edges over nx/ny (terminals first, Steiner points appended).
+// g[v] = min over the splits of 'mask' (the part holding its lowest bit, and
+// the rest) of S[e][v] + S[mask ^ e][v].
+// Rows are n entries long, n a multiple of 8 (grid points padded).
+template <class T>
+__attribute__((target_clones("avx2", "default")))
+void dw_splits(const T* __restrict S, int n, int mask, T* __restrict g) {
+    const int low = mask & -mask, rest = mask ^ low;
+    for (int v = 0; v < n; v++) g[v] = std::numeric_limits<T>::max();
+    for (int sub = (rest - 1) & rest;; sub = (sub - 1) & rest) {
+        const int e = low | sub, f = mask ^ e;
+        const T* __restrict se = S + (size_t)e * n;
+        const T* __restrict sf = S + (size_t)f * n;
+        for (int v0 = 0; v0 < n; v0 += 8)
+#pragma GCC unroll 8
+            for (int v = v0; v < v0 + 8; v++) {
+                const T c = se[v] + sf[v];
+                g[v] = c < g[v] ? c : g[v];
+            }
+        if (sub == 0) break;
+    }
+}
+
+// Exact rectilinear Steiner minimal tree of 5..9 terminals by the
+// Dreyfus-Wagner recurrence (Dreyfus and Wagner, Networks 1971) over the
+// Hanan grid of the terminals (Hanan 1966: some minimal tree uses only grid
+// points), with L1 distances in gcell-length coordinates.  The last terminal
+// is the root; for every subset D of the others and grid point v,
+//   S(D, v) = min_u dist(v, u) + g(D, u),
+//   g(D, u) = min over splits D = E + F (E holding D's lowest terminal) of
+//             S(E, u) + S(F, u),
+// with S({t}, v) = dist(v, t).  The min over u is an L1 distance transform
+// on the grid (a forward and a backward sweep per axis), so a subset costs
+// O(splits * grid) instead of O(grid^2).  The tree is read back top-down:
+// at (D, v) the first grid point u (in grid order) with dist(v, u) + g(D, u)
+// = S(D, v), then the first split (decreasing subset order) attaining
+// g(D, u); so the tree is deterministic.  Steiner points of degree two on a
+// shortest path between their neighbours are dropped.  Returns the edges
+// over nx/ny (terminals first, Steiner points appended).
+template <class T>
+void rsmt_dw(const Coords& C, std::vector<int>& nx, std::vector<int>& ny, std::vector<std::pair<int, int>>& edges) {
+    const int k = (int)nx.size();
+    int hx[9], hy[9];
+    for (int i = 0; i < k; i++) { hx[i] = nx[i]; hy[i] = ny[i]; }
+    std::sort(hx, hx + k);
+    std::sort(hy, hy + k);
+    const int HX = (int)(std::unique(hx, hx + k) - hx), HY = (int)(std::unique(hy, hy + k) - hy);
+    const int n = HX * HY;
+    T cx[9], cy[9];
+    for (int i = 0; i < HX; i++) cx[i] = (T)C.drx[hx[i]];
+    for (int j = 0; j < HY; j++) cy[j] = (T)C.dry[hy[j]];
+    auto dist = [&](int v, int u) {
+        const int vi = v / HY, vj = v % HY, ui = u / HY, uj = u % HY;
+        return (cx[vi] > cx[ui] ? cx[vi] - cx[ui] : cx[ui] - cx[vi]) +
+               (cy[vj] > cy[uj] ? cy[vj] - cy[uj] : cy[uj] - cy[vj]);
+    };
+    int term[9];
+    for (int t = 0; t < k; t++)
+        term[t] = (int)(std::lower_bound(hx, hx + HX, nx[t]) - hx) * HY +
+                  (int)(std::lower_bound(hy, hy + HY, ny[t]) - hy);
+    const int m = k - 1, M = 1 << m;
+    const int np = (n + 7) & ~7;   // row stride; padding entries stay 0
+    thread_local std::vector<T> S, G;
+    S.assign((size_t)M * np, T(0));
+    G.resize((size_t)M * np);
+    for (int t = 0; t < m; t++) {
+        T* st = &S[(size_t)(1 << t) * np];
+        for (int v = 0; v < n; v++) st[v] = dist(v, term[t]);
+    }
+    for (int mask = 3; mask < M; mask++) {
+        if ((mask & (mask - 1)) == 0) continue;
+        T* g = &G[(size_t)mask * np];
+        T* sm = &S[(size_t)mask * np];
+        dw_splits<T>(S.data(), np, mask, g);
+        // L1 distance transform: along x (j fixed), then along y (i fixed)
+        for (int v = 0; v < n; v++) sm[v] = g[v];
+        for (int i = 1; i < HX; i++) {
+            const T dx = cx[i] - cx[i - 1];
+            for (int j = 0; j < HY; j++) sm[i * HY + j] = std::min(sm[i * HY + j], sm[(i - 1) * HY + j] + dx);
+        }
+        for (int i = HX - 2; i >= 0; i--) {
+            const T dx = cx[i + 1] - cx[i];
+            for (int j = 0; j < HY; j++) sm[i * HY + j] = std::min(sm[i * HY + j], sm[(i + 1) * HY + j] + dx);
+        }
+        // (rows i independent: the inner loop carries no dependency)
+        for (int j = 1; j < HY; j++) {
+            const T dy = cy[j] - cy[j - 1];
+            for (int i = 0; i < HX; i++) sm[i * HY + j] = std::min(sm[i * HY + j], sm[i * HY + j - 1] + dy);
+        }
+        for (int j = HY - 2; j >= 0; j--) {
+            const T dy = cy[j + 1] - cy[j];
+            for (int i = 0; i < HX; i++) sm[i * HY + j] = std::min(sm[i * HY + j], sm[i * HY + j + 1] + dy);
+        }
+    }
+    // Tree from the recurrence: grid point -> node (terminals keep their ids).
+    int node_of[81];
+    for (int v = 0; v < n; v++) node_of[v] = -1;
+    for (int t = 0; t < k; t++) node_of[term[t]] = t;
+    auto node = [&](int v) {
+        if (node_of[v] < 0) {
+            node_of[v] = (int)nx.size();
+            nx.push_back(hx[v / HY]);
+            ny.push_back(hy[v % HY]);
+        }
+        return node_of[v];
+    };
+    std::vector<std::pair<int, int>> es;
+    struct Item { int mask, v; };
+    Item stack[64];
+    int sp = 0;
+    stack[sp++] = {M - 1, term[k - 1]};
+    while (sp > 0) {
+        const Item it = stack[--sp];
+        if ((it.mask & (it.mask - 1)) == 0) {
+            const int t = __builtin_ctz(it.mask);
+            if (term[t] != it.v) es.emplace_back(node(it.v), t);
+            continue;
+        }
+        const T* g = &G[(size_t)it.mask * np];
+        const T target = S[(size_t)it.mask * np + it.v];
+        int u = 0;
+        while (dist(it.v, u) + g[u] != target) u++;
+        if (u != it.v) es.emplace_back(node(it.v), node(u));
+        const int low = it.mask & -it.mask, rest = it.mask ^ low;
+        int e = low;
+        for (int sub = (rest - 1) & rest;; sub = (sub - 1) & rest) {
+            e = low | sub;
+            if (S[(size_t)e * np + u] + S[(size_t)(it.mask ^ e) * np + u] == g[u]) break;
+            if (sub == 0) break;
+        }
+        stack[sp++] = {it.mask ^ e, u};
+        stack[sp++] = {e, u};
+    }
+    // Drop Steiner points of degree two that lie on a shortest path between
+    // their neighbours (and Steiner leaves, which only zero-length edges make).
+    const int nn = (int)nx.size();
+    std::vector<std::vector<int>> adj(nn);
+    for (auto& e : es) {
+        adj[e.first].push_back(e.second);
+        adj[e.second].push_back(e.first);
+    }
+    auto D = [&](int a, int b) { return C.dist(nx[a], ny[a], nx[b], ny[b]); };
+    bool changed = true;
+    while (changed) {
+        changed = false;
+        for (int s = k; s < nn; s++) {
+            auto& a = adj[s];
+            if (a.empty() || a.size() > 2) continue;
+            if (a.size() == 2 && D(a[0], s) + D(s, a[1]) != D(a[0], a[1])) continue;
+            for (int c : a) adj[c].erase(std::find(adj[c].begin(), adj[c].end(), s));
+            if (a.size() == 2) {
+                adj[a[0]].push_back(a[1]);
+                adj[a[1]].push_back(a[0]);
+            }
+            a.clear();
+            changed = true;
+        }
+    }
+    edges.clear();
+    for (int a = 0; a < nn; a++)
+        for (int b : adj[a])
+            if (a < b) edges.emplace_back(a, b);
+}
+
 // Steiner tree over distinct 2-D positions.
 void steiner_tree(const Coords& C, const std::vector<int>& pts, Tree2D& t) {
     const int n = (int)pts.size();
     std::vector<int> nx(n), ny(n);
     for (int i = 0; i < n; i++) { nx[i] = pts[i] / C.Y; ny[i] = pts[i] % C.Y; }
     std::vector<std::pair<int, int>> edges;
-    if (n == 4) {
+    const long long span = C.drx.back() + C.dry.back();
+    if (n >= 5 && n <= 9) {
+        if (span < INT_MAX / 32) rsmt_dw<int>(C, nx, ny, edges);
+        else rsmt_dw<long long>(C, nx, ny, edges);
+    } else if (n == 4) {
         // exact: spanning tree over the pins and the chosen Steiner points
         rsmt4(C, nx, ny);
         std::vector<std::vector<int>> adj(nx.size());
-        const long long span = C.drx.back() + C.dry.back();
         if (span < INT_MAX / 2) prim<int>(C, nx, ny, adj);
         else prim<long long>(C, nx, ny, adj);
         for (int a = 0; a < (int)adj.size(); a++)
@@ -242,7 +423,6 @@ void steiner_tree(const Coords& C, const std::vector<int>& pts, Tree2D& t) {
                 if (a < b) edges.emplace_back(a, b);
     } else if (n >= 2) {
         std::vector<std::vector<int>> adj(n);
-        const long long span = C.drx.back() + C.dry.back();
         if (span < INT_MAX / 2) prim<int>(C, nx, ny, adj);
         else prim<long long>(C, nx, ny, adj);
         // Steiner improvement: median stars on pairs of incident edges.
@@ -429,6 +609,31 @@ void split_net(const Coords& C, const Grid& g, const std::vector<int>& pins, std
 }
 
 } // namespace
+
+void steiner_of_points(const std::vector<int>& x, const std::vector<int>& y, const std::vector<int>& xlen,
+                       const std::vector<int>& ylen, std::vector<std::pair<int, int>>& nodes,
+                       std::vector<std::pair<int, int>>& edges) {
+    const int X = (int)xlen.size() + 1, Y = (int)ylen.size() + 1;
+    Coords C{X, Y, std::vector<long long>(X, 0), std::vector<long long>(Y, 0)};
+    for (int i = 0; i + 1 < X; i++) C.drx[i + 1] = C.drx[i] + xlen[i];
+    for (int i = 0; i + 1 < Y; i++) C.dry[i + 1] = C.dry[i] + ylen[i];
+    std::vector<int> pts(x.size());
+    for (size_t i = 0; i < x.size(); i++) {
+        if (x[i] < 0 || x[i] >= X || y[i] < 0 || y[i] >= Y) throw std::invalid_argument("point outside the grid");
+        pts[i] = x[i] * Y + y[i];
+    }
+    std::sort(pts.begin(), pts.end());
+    if (std::adjacent_find(pts.begin(), pts.end()) != pts.end())
+        throw std::invalid_argument("points must be distinct");
+    Tree2D t;
+    steiner_tree(C, pts, t);
+    nodes.clear();
+    edges.clear();
+    for (int p : t.pos) nodes.emplace_back(p / Y, p % Y);
+    for (int i = 0; i < (int)t.pos.size(); i++)
+        for (int k = t.adj.off[i]; k < t.adj.off[i + 1]; k++)
+            if (i < t.adj.nb[k]) edges.emplace_back(i, t.adj.nb[k]);
+}
 
 void build_subnets(const Design& d, const NetPins& np, const Grid& g, Subnets& s, int threads) {
     const int64_t N = d.num_nets();
