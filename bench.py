@@ -272,7 +272,10 @@ def summarize_lds(root):
 def cost_prefix_sweeps(design, reps=20):
     """Full-grid cost update + prefix sums in both forms, on the benchmark grid
     and on the largest grid the generator defines (12 metal layers,
-    2500x2500); GB/s over 28 algorithmic bytes per cell."""
+    2500x2500).  GB/s over the algorithmic bytes per cell: 28 for the plain
+    form's three kernels (cost update: read capacity and demand, write wire
+    and via cost; prefix sums: read wire cost, write the FP64 sums), 24 for
+    the fused kernel, which does not read the wire cost back."""
     import numpy as np
     from instantgr_amd import gpu
     out = []
@@ -287,9 +290,9 @@ def cost_prefix_sweeps(design, reps=20):
             r = gpu.Router(L, X, Y, 1, 1.0, 4.0, np.ones(L), np.ones(X - 1, np.int32), np.ones(Y - 1, np.int32), cap)
         cells = r.n
         row = {"grid": name, "cells": cells}
-        for form, key in ((gpu.FORM_PLAIN, "plain"), (gpu.FORM_BATCHED, "fused")):
+        for form, key, per in ((gpu.FORM_PLAIN, "plain", 28.0), (gpu.FORM_BATCHED, "fused", 24.0)):
             ms = r.cost_sweep_ms(form, reps)
-            gbs = 28.0 * cells / (ms * 1e-3) / 1e9
+            gbs = per * cells / (ms * 1e-3) / 1e9
             row.update({key + "_us": round(ms * 1e3, 2), key + "_GBs": round(gbs, 1),
                         key + "_frac": round(gbs / 8000.0, 4)})
         r.close()

@@ -226,6 +226,7 @@ typedef struct igr_tuning {
     int sched_kind;     /* scheduler rounds after the first of a batch: 1 active-list (work follows the
                            active nets), 2 release-driven (work follows the released cells); built-in
                            choice 2 from 1.5M scheduled nets */
+    int presum_k;       /* plain form's prefix-sum kernel: cells per thread, 4, 8 or 16 */
 } igr_tuning;
 int igr_set_tuning(igr_ctx* ctx, const igr_tuning* t);
 /* The values in force (built-in choices resolved, per-pass choices as 0). */

@@ -11,47 +11,172 @@ namespace {
 
 constexpr int TPB = 256;
 
-// Decode a device cell offset into (layer, track, position, track length).
-struct CellPos {
-    int l, track, pos, len;
-};
-__device__ __forceinline__ CellPos decode(const GridDev& g, int64_t i) {
-    CellPos c;
-    c.l = (int)(i / g.XYc);
-    int64_t r = i - (int64_t)c.l * g.XYc;
-    c.len = layer_dir(c.l, g.DIR) ? g.Y : g.X;
-    c.track = (int)(r / c.len);
-    c.pos = (int)(r - (int64_t)c.track * c.len);
-    return c;
+// Track of a two-dimensional launch (blockIdx.y = track in layer order):
+// its layer, length and first cell.  blockIdx.y is uniform, so this is scalar
+// work; no per-cell division.
+__device__ __forceinline__ void track_at(const GridDev& g, int t, int& l, int& len, int64_t& base) {
+    l = 0;
+    while (l < g.L - 1 && t >= g.tbase[l + 1]) l++;
+    len = layer_dir(l, g.DIR) ? g.Y : g.X;
+    base = (int64_t)l * g.XYc + (int64_t)(t - g.tbase[l]) * len;
+}
+
+// N consecutive cells of a track per thread (N a multiple of 4), with 16-byte
+// accesses when the track length is a multiple of 4, which keeps every track
+// 16-byte aligned.  All of a thread's loads are issued before any use, so a
+// wave keeps N * 256 bytes per array in flight.
+template <int N, typename T, typename V4>
+__device__ __forceinline__ void ldn(const T* __restrict__ a, int64_t base, int p0, int len, T (&v)[N]) {
+    if ((len & 3) == 0) {
+#pragma unroll
+        for (int j = 0; j < N; j += 4) {
+            if (p0 + j < len) {
+                const V4 q = *reinterpret_cast<const V4*>(a + base + p0 + j);
+                v[j] = q.x; v[j + 1] = q.y; v[j + 2] = q.z; v[j + 3] = q.w;
+            } else {
+                v[j] = v[j + 1] = v[j + 2] = v[j + 3] = T(0);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < N; j++) v[j] = p0 + j < len ? a[base + p0 + j] : T(0);
+    }
+}
+template <int N>
+__device__ __forceinline__ void stn(float* __restrict__ a, int64_t base, int p0, int len, const float (&v)[N]) {
+    if ((len & 3) == 0) {
+#pragma unroll
+        for (int j = 0; j < N; j += 4)
+            if (p0 + j < len) *reinterpret_cast<float4*>(a + base + p0 + j) = make_float4(v[j], v[j + 1], v[j + 2], v[j + 3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < N; j++)
+            if (p0 + j < len) a[base + p0 + j] = v[j];
+    }
+}
+
+constexpr int CPT = 16;   // cells per thread of the cost kernels
+constexpr int CTPB = 64;  // their workgroup: one wave, 1024 cells of a track
+
+// Copies the exp table into LDS (all threads of the block take part; the
+// caller synchronises before use).
+__device__ __forceinline__ void load_exptab(const GridDev& g, float* s_exp) {
+    for (int i = threadIdx.x; i < EXPTAB_N; i += blockDim.x) s_exp[i] = g.exptab[i];
+}
+
+// Gcell lengths el[p0 .. p0 + CPT) of a track's edges (0 past its last edge,
+// which has no length: the track has len - 1 edges).
+__device__ __forceinline__ void load_elen(const int* __restrict__ el, int p0, int len, int (&e)[CPT]) {
+    if (p0 + CPT <= len - 1) {
+#pragma unroll
+        for (int j = 0; j < CPT; j += 4) {
+            const int4 q = *reinterpret_cast<const int4*>(el + p0 + j);
+            e[j] = q.x; e[j + 1] = q.y; e[j + 2] = q.z; e[j + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < CPT; j++) e[j] = p0 + j < len - 1 ? el[p0 + j] : 0;
+    }
+}
+
+template <bool TAB>
+__device__ __forceinline__ void wcost_cells(const GridDev& g, const float (&c)[CPT], const float (&d)[CPT], int p0,
+                                            int len, double w, const int (&e)[CPT], const float* tab,
+                                            float (&o)[CPT]) {
+#pragma unroll
+    for (int j = 0; j < CPT; j++) {
+        const float a1 = (float)inc_one<TAB>(c[j], d[j], 1.0f, w, tab);
+        const float wl = (float)((double)a1 + g.unit_wire * (double)e[j]);
+        o[j] = p0 + j < len - 1 ? wl : a1;
+    }
 }
 
 // wcost = f32(inc_cost(+1)) then f32(wcost + unit_wire * edge length) for all
-// but the last cell of a track (src/graph.hpp:40-47).
-__global__ void __launch_bounds__(TPB) k_update_wcost(GridDev g) {
-    int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (i >= (int64_t)g.L * g.XYc) return;
-    CellPos c = decode(g, i);
-    float wv = (float)inc_cost(g.cap[i], g.demand[i], 1.0, g.w[c.l]);
-    if (c.pos < c.len - 1) {
-        int el = layer_dir(c.l, g.DIR) ? g.ylen[c.pos] : g.xlen[c.pos];
-        wv = (float)((double)wv + g.unit_wire * (double)el);
+// but the last cell of a track (src/graph.hpp:40-47).  The exponents come
+// from the LDS table when every cell of the wave allows it (quarter_grid),
+// else they are computed; the values are the same either way.
+__global__ void __launch_bounds__(CTPB) k_update_wcost(GridDev g) {
+    __shared__ float s_exp[EXPTAB_N];
+    int l, len;
+    int64_t base;
+    track_at(g, blockIdx.y, l, len, base);
+    const int p0 = (blockIdx.x * CTPB + threadIdx.x) * CPT;
+    float c[CPT], d[CPT], o[CPT];
+    int e[CPT];
+    if (p0 < len) {
+        ldn<CPT, float, float4>(g.cap, base, p0, len, c);
+        ldn<CPT, float, float4>(g.demand, base, p0, len, d);
+        load_elen(layer_dir(l, g.DIR) ? g.ylen : g.xlen, p0, len, e);
     }
-    g.wcost[i] = wv;
+    load_exptab(g, s_exp);
+    __syncthreads();
+    if (p0 >= len) return;
+    const double w = g.w[l];
+    bool q = true;
+#pragma unroll
+    for (int j = 0; j < CPT; j++) q = q && quarter_grid(c[j], d[j]);
+    if (__all(q)) wcost_cells<true>(g, c, d, p0, len, w, e, s_exp, o);
+    else wcost_cells<false>(g, c, d, p0, len, w, e, s_exp, o);
+    stn<CPT>(g.wcost, base, p0, len, o);
+}
+
+// Interior cells take the half increments of the cell and the one before;
+// the track's first cell the whole increment of its edge, the last cell the
+// whole increment of the edge before it (only the threads holding a track
+// end take that branch).
+template <bool TAB>
+__device__ __forceinline__ void vcost_cells(const GridDev& g, const float (&c)[CPT], const float (&d)[CPT], float cp,
+                                            float dp, int p0, int len, double w, const float* tab, float (&o)[CPT]) {
+    double hp = inc_one<TAB>(cp, dp, 0.5f, w, tab);   // half increment of the cell before
+    float vv[CPT];
+#pragma unroll
+    for (int j = 0; j < CPT; j++) {
+        const double h = inc_one<TAB>(c[j], d[j], 0.5f, w, tab);
+        vv[j] = (float)(h + hp);
+        hp = h;
+    }
+    if (p0 == 0 || p0 + CPT > len - 1) {
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+            const int p = p0 + j;
+            if (p == 0) vv[j] = (float)inc_one<TAB>(c[j], d[j], 1.0f, w, tab);
+            else if (p == len - 1) vv[j] = (float)inc_one<TAB>(j ? c[j - 1] : cp, j ? d[j - 1] : dp, 1.0f, w, tab);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < CPT; j++) o[j] = (float)((double)vv[j] + g.unit_via);
 }
 
 // Via cost: half the cost increment of the two track edges meeting at the
 // cell, the whole increment at a track end, plus unit_via
-// (src/graph.hpp:52-71; cross_points is identically zero).
-__global__ void __launch_bounds__(TPB) k_update_vcost(GridDev g) {
-    int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (i >= (int64_t)g.L * g.XYc) return;
-    CellPos c = decode(g, i);
-    double wl = g.w[c.l];
-    float vv;
-    if (c.pos == 0) vv = (float)inc_cost(g.cap[i], g.demand[i], 1.0, wl);
-    else if (c.pos == c.len - 1) vv = (float)inc_cost(g.cap[i - 1], g.demand[i - 1], 1.0, wl);
-    else vv = (float)(inc_cost(g.cap[i], g.demand[i], 0.5, wl) + inc_cost(g.cap[i - 1], g.demand[i - 1], 0.5, wl));
-    g.vcost[i] = (float)((double)vv + g.unit_via);
+// (src/graph.hpp:52-71; cross_points is identically zero).  A thread's cells
+// and the one before them give the half increments it needs.
+__global__ void __launch_bounds__(CTPB) k_update_vcost(GridDev g) {
+    __shared__ float s_exp[EXPTAB_N];
+    int l, len;
+    int64_t base;
+    track_at(g, blockIdx.y, l, len, base);
+    const int p0 = (blockIdx.x * CTPB + threadIdx.x) * CPT;
+    float c[CPT], d[CPT], o[CPT];
+    float cp = 0.0f, dp = 0.0f;
+    if (p0 < len) {
+        if (p0 > 0) {
+            cp = g.cap[base + p0 - 1];
+            dp = g.demand[base + p0 - 1];
+        }
+        ldn<CPT, float, float4>(g.cap, base, p0, len, c);
+        ldn<CPT, float, float4>(g.demand, base, p0, len, d);
+    }
+    load_exptab(g, s_exp);
+    __syncthreads();
+    if (p0 >= len) return;
+    const double w = g.w[l];
+    bool q = quarter_grid(cp, dp);
+#pragma unroll
+    for (int j = 0; j < CPT; j++) q = q && quarter_grid(c[j], d[j]);
+    if (__all(q)) vcost_cells<true>(g, c, d, cp, dp, p0, len, w, s_exp, o);
+    else vcost_cells<false>(g, c, d, cp, dp, p0, len, w, s_exp, o);
+    stn<CPT>(g.vcost, base, p0, len, o);
 }
 
 // Track index -> (layer, first cell, length).
@@ -67,25 +192,58 @@ __device__ __forceinline__ void track_of(const GridDev& g, int t, int64_t& base,
 }
 
 // presum[p] = Sklansky-exclusive prefix of wcost along the track, FP64
-// (src/graph.hpp:405-430).  One workgroup per track.
+// (src/graph.hpp:405-430).  One workgroup per track (blockIdx.x = track) of
+// just enough waves for its cells, K cells per thread: 16-byte loads of
+// wcost[p0 .. p0+3], the element before p0 from the previous lane.
 template <int K>
-__global__ void __launch_bounds__(TPB) k_presum(GridDev g) {
-    __shared__ double lds[TPB / 64];
+__global__ void __launch_bounds__(1024) k_presum(GridDev g) {
+    __shared__ double lds[16];
+    int l, len;
     int64_t base;
-    int len;
-    track_of(g, blockIdx.x, base, len);
+    track_at(g, blockIdx.x, l, len, base);
+    const int nw = blockDim.x >> 6;
+    const int p0 = threadIdx.x * K;
+    float w[K];
+    if constexpr (K % 4 == 0) {
+        if ((len & 3) == 0) {
+#pragma unroll
+            for (int j = 0; j < K; j += 4) {
+                if (p0 + j < len) {
+                    const float4 q = *reinterpret_cast<const float4*>(g.wcost + base + p0 + j);
+                    w[j] = q.x; w[j + 1] = q.y; w[j + 2] = q.z; w[j + 3] = q.w;
+                } else {
+                    w[j] = w[j + 1] = w[j + 2] = w[j + 3] = 0.0f;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < K; j++) w[j] = p0 + j < len ? g.wcost[base + p0 + j] : 0.0f;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < K; j++) w[j] = p0 + j < len ? g.wcost[base + p0 + j] : 0.0f;
+    }
+    // element p0 - 1: the previous lane's last, or a load at a wave's first lane
+    float before = __shfl_up(w[K - 1], 1, 64);
+    if ((threadIdx.x & 63) == 0) before = (p0 >= 1 && p0 - 1 < len) ? g.wcost[base + p0 - 1] : 0.0f;
     double s[K];
 #pragma unroll
     for (int j = 0; j < K; j++) {
-        int p = threadIdx.x * K + j;
-        s[j] = (p >= 1 && p < len) ? (double)g.wcost[base + p - 1] : 0.0;
+        const int p = p0 + j;
+        s[j] = (p >= 1 && p < len) ? (double)(j ? w[j - 1] : before) : 0.0;
     }
-    block_sklansky<double, K, TPB / 64>(s, lds);
+    block_sklansky_rt<double, K, 16>(s, lds, nw);
+    if constexpr (K % 2 == 0) {
+        if ((len & 1) == 0) {
 #pragma unroll
-    for (int j = 0; j < K; j++) {
-        int p = threadIdx.x * K + j;
-        if (p < len) g.presum[base + p] = s[j];
+            for (int j = 0; j < K; j += 2)
+                if (p0 + j < len) *reinterpret_cast<double2*>(g.presum + base + p0 + j) = make_double2(s[j], s[j + 1]);
+            return;
+        }
     }
+#pragma unroll
+    for (int j = 0; j < K; j++)
+        if (p0 + j < len) g.presum[base + p0 + j] = s[j];
 }
 
 // In-place exclusive int prefix of the difference array along each track
@@ -111,12 +269,14 @@ __global__ void __launch_bounds__(TPB) k_scan_int(GridDev g, int* a) {
 }
 
 // demand += covered count, timestamp = stamp where the count is positive
-// (src/graph.hpp:118-132).
+// (src/graph.hpp:118-132).  Two-dimensional launch: blockIdx.y = track.
 __global__ void __launch_bounds__(TPB) k_commit_edges(GridDev g, int stamp) {
-    int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (i >= (int64_t)g.L * g.XYc) return;
-    CellPos c = decode(g, i);
-    if (c.pos + 1 < c.len) {
+    int l, len;
+    int64_t base;
+    track_at(g, blockIdx.y, l, len, base);
+    const int p = blockIdx.x * TPB + threadIdx.x;
+    if (p + 1 < len) {
+        const int64_t i = base + p;
         int v = g.pre[i + 1];
         if (v > 0) {
             g.demand[i] += (float)v;
@@ -156,12 +316,15 @@ __global__ void __launch_bounds__(TPB) k_via_commit(GridDev g, const int* nets, 
 }
 
 // of = 1 on track edges whose demand reaches capacity + threshold
-// (src/graph.hpp:432-438).
+// (src/graph.hpp:432-438).  Two-dimensional launch: blockIdx.y = track.
 __global__ void __launch_bounds__(TPB) k_mark_edges(GridDev g, int th, int* of) {
-    int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (i >= (int64_t)g.L * g.XYc) return;
-    CellPos c = decode(g, i);
-    of[i] = (c.pos + 1 < c.len && g.cap[i] + (float)th <= g.demand[i]) ? 1 : 0;
+    int l, len;
+    int64_t base;
+    track_at(g, blockIdx.y, l, len, base);
+    const int p = blockIdx.x * TPB + threadIdx.x;
+    if (p >= len) return;
+    const int64_t i = base + p;
+    of[i] = (p + 1 < len && g.cap[i] + (float)th <= g.demand[i]) ? 1 : 0;
 }
 
 // A net is ripped when more than 10 full edges lie under its wires
@@ -240,6 +403,11 @@ __global__ void __launch_bounds__(TPB) k_from_ref(GridDev g, const T* src, T* ds
     dst[gidx(g, l, x, y)] = src[r];
 }
 
+__global__ void k_exptab(float* t) {
+    const int i = blockIdx.x * TPB + threadIdx.x;
+    if (i < EXPTAB_N) t[i] = igr_expf((float)(i + EXPTAB_LO) * 0.25f);
+}
+
 __global__ void k_expf(const float* in, float* out, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (i < n) out[i] = igr_expf(in[i]);
@@ -268,18 +436,37 @@ int tracks(const GridDev& g) {
         HIP_OK(hipGetLastError());                                                                     \
     } while (0)
 
+// Grid of a two-dimensional launch over every track, 'per' cells per thread.
+dim3 track_grid(const GridDev& g, int tpb, int per) {
+    const int ml = std::max(g.X, g.Y);
+    if (g.tbase[g.L] > 65535) throw std::runtime_error("more than 65535 tracks");
+    return dim3((unsigned)((ml + tpb * per - 1) / (tpb * per)), (unsigned)g.tbase[g.L]);
+}
+
 void launch_update_cost(igr_ctx* c) {
     int64_t n = c->cells();
     KScope ks(c, K_COST, 16.0 * n);   // read cap, demand; write wcost, vcost
-    k_update_wcost<<<blocks(n), TPB, 0, c->stream>>>(c->g);
+    k_update_wcost<<<track_grid(c->g, CTPB, CPT), CTPB, 0, c->stream>>>(c->g);
     HIP_OK(hipGetLastError());
-    k_update_vcost<<<blocks(n), TPB, 0, c->stream>>>(c->g);
+    k_update_vcost<<<track_grid(c->g, CTPB, CPT), CTPB, 0, c->stream>>>(c->g);
     HIP_OK(hipGetLastError());
 }
 
 void launch_presum(igr_ctx* c) {
     KScope ks(c, K_PRESUM, 12.0 * c->cells());   // read wcost f32, write presum f64
-    DISPATCH_K(std::max(c->g.X, c->g.Y), k_presum, tracks(c->g), c->stream, c->g);
+    // 16 cells per thread (tunable); the workgroup is the fewest waves
+    // covering the longest track.
+    const int ml = std::max(c->g.X, c->g.Y);
+    const int K = c->tune.presum_k ? c->tune.presum_k : 16;
+    const int B = ((ml + K - 1) / K + 63) / 64 * 64;
+    if (B > 1024) throw std::runtime_error("track longer than the prefix-sum kernel's tile");
+    switch (K) {
+    case 4: k_presum<4><<<tracks(c->g), B, 0, c->stream>>>(c->g); break;
+    case 8: k_presum<8><<<tracks(c->g), B, 0, c->stream>>>(c->g); break;
+    case 16: k_presum<16><<<tracks(c->g), B, 0, c->stream>>>(c->g); break;
+    default: throw std::invalid_argument("presum_k must be 4, 8 or 16");
+    }
+    HIP_OK(hipGetLastError());
 }
 
 void launch_wire_commit(igr_ctx* c, int stamp) {
@@ -288,7 +475,7 @@ void launch_wire_commit(igr_ctx* c, int stamp) {
     KScope ks(c, K_WIRE_COMMIT, 16.0 * c->cells());
     DISPATCH_K(std::max(c->g.X, c->g.Y), k_scan_int, tracks(c->g), c->stream, c->g, c->g.pre);
     int64_t n = c->cells();
-    k_commit_edges<<<blocks(n), TPB, 0, c->stream>>>(c->g, stamp);
+    k_commit_edges<<<track_grid(c->g, TPB, 1), TPB, 0, c->stream>>>(c->g, stamp);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemsetAsync(c->g.pre, 0, sizeof(int) * n, c->stream));
 }
@@ -345,8 +532,7 @@ void launch_from_ref(igr_ctx* c, const void* src, void* dst, int elem_bytes) {
 }
 
 void launch_ripup_mark(igr_ctx* c, int th, uint8_t* d_is_of, int* d_of) {
-    int64_t n = c->cells();
-    k_mark_edges<<<blocks(n), TPB, 0, c->stream>>>(c->g, th, d_of);
+    k_mark_edges<<<track_grid(c->g, TPB, 1), TPB, 0, c->stream>>>(c->g, th, d_of);
     HIP_OK(hipGetLastError());
     DISPATCH_K(std::max(c->g.X, c->g.Y), k_scan_int, tracks(c->g), c->stream, c->g, d_of);
     k_mark_nets<<<blocks(c->nets.n), TPB, 0, c->stream>>>(c->g, c->nets.n, c->nets.rptr.p, d_of, d_is_of);
@@ -366,6 +552,11 @@ void launch_congestion(igr_ctx* c, float* d_xsum, float* d_ysum) {
     HIP_OK(hipStreamSynchronize(c->stream));
 }
 
+void launch_exptab(float* t, hipStream_t s) {
+    k_exptab<<<blocks(EXPTAB_N), TPB, 0, s>>>(t);
+    HIP_OK(hipGetLastError());
+}
+
 void launch_expf(const float* in, float* out, int64_t n, hipStream_t s) {
     k_expf<<<blocks(n), TPB, 0, s>>>(in, out, n);
     HIP_OK(hipGetLastError());
@@ -373,10 +564,23 @@ void launch_expf(const float* in, float* out, int64_t n, hipStream_t s) {
 
 namespace {
 // Plain 16-byte-vector copy: the bandwidth reference and the calibration
-// kernel for the memory counters (bytes read = bytes written = n * 16).
+// kernel for the memory counters (bytes read = bytes written = n * 16).  Each
+// thread moves four 16-byte vectors, a wave's loads covering 4 KiB, all four
+// loads issued before the stores.
 __global__ void __launch_bounds__(TPB) k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst,
                                                     int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) dst[i] = src[i];
+    const int64_t i0 = (int64_t)blockIdx.x * TPB * 4 + threadIdx.x;
+    float4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int64_t i = i0 + (int64_t)k * TPB;
+        if (i < n) v[k] = src[i];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int64_t i = i0 + (int64_t)k * TPB;
+        if (i < n) dst[i] = v[k];
+    }
 }
 } // namespace
 
@@ -386,7 +590,7 @@ double copy_probe(int64_t bytes, int reps, hipStream_t s) {
     DevBuf<float4> a, b;
     a.zero(n, s);
     b.alloc(n);
-    const unsigned grid = (unsigned)std::min<int64_t>((n + TPB - 1) / TPB, 256 * 64);
+    const unsigned grid = (unsigned)((n + 4 * TPB - 1) / (4 * TPB));
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));

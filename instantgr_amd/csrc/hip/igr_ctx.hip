@@ -96,6 +96,9 @@ int igr_create(const igr_grid_desc* d, igr_ctx** out) {
         g.tbase[0] = 0;
         for (int l = 0; l < g.L; l++) g.tbase[l + 1] = g.tbase[l] + (layer_dir(l, g.DIR) ? g.X : g.Y);
         c->track_stats.zero(2, c->stream);
+        c->exptab.alloc(EXPTAB_N);
+        launch_exptab(c->exptab.p, c->stream);
+        g.exptab = c->exptab.p;
         launch_from_ref(c, tmp.p, c->cap.p, 4);
         HIP_OK(hipStreamSynchronize(c->stream));
     });
@@ -366,21 +369,55 @@ int igr_pattern_route(igr_ctx* c, const int* order, int64_t n, int form, igr_pas
         std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
         for (int64_t i = n - 1; i >= 0; i--) list[pos[nb - 1 - batch_of[i]]++] = order[i];
         std::vector<int64_t> off(n);
-        int64_t max_nodes = 0;
-        for (int b = 0; b < nb; b++)
-            max_nodes = std::max(max_nodes, scratch_offsets(c, list.data() + cnt[b], cnt[b + 1] - cnt[b],
-                                                            off.data() + cnt[b]));
-        ensure_scratch(c, max_nodes);
-        c->scratch.net_list.upload(list.data(), n, c->stream);
-        c->scratch.scr_off.upload(off.data(), n, c->stream);
-        std::vector<double> lbytes(nb, 0.0);
-        for (int b = 0; b < nb; b++)
-            for (int64_t i = cnt[b]; i < cnt[b + 1]; i++) lbytes[b] += c->nets.h_lbytes[list[i]];
         if (form == IGR_FORM_PLAIN) {
+            int64_t max_nodes = 0;
+            for (int b = 0; b < nb; b++)
+                max_nodes = std::max(max_nodes, scratch_offsets(c, list.data() + cnt[b], cnt[b + 1] - cnt[b],
+                                                                off.data() + cnt[b]));
+            ensure_scratch(c, max_nodes);
+            c->scratch.net_list.upload(list.data(), n, c->stream);
+            c->scratch.scr_off.upload(off.data(), n, c->stream);
+            std::vector<double> lbytes(nb, 0.0);
+            for (int b = 0; b < nb; b++)
+                for (int64_t i = cnt[b]; i < cnt[b + 1]; i++) lbytes[b] += c->nets.h_lbytes[list[i]];
             for (int b = 0; b < nb; b++)
                 plain_batch(c, c->scratch.net_list.p + cnt[b], c->scratch.scr_off.p + cnt[b], cnt[b + 1] - cnt[b],
                             lbytes[b]);
         } else {
+            // Each batch's nets grouped by node-count class, one launch per
+            // class: the order of nets inside a batch changes no result (their
+            // footprints are disjoint and their demand updates integer adds).
+            // Nets above the largest class (none under the 12-pin subnet cap)
+            // take the thread-per-net kernel.
+            constexpr int NCLS = 5;
+            std::vector<int> wl(n);
+            std::vector<int64_t> co((size_t)nb * NCLS + 1, 0);
+            std::vector<double> cbytes((size_t)nb * NCLS, 0.0);
+            {
+                std::vector<int> cls(n);
+                for (int64_t i = 0; i < n; i++) {
+                    const int k = lshape_wave_class(
+                        (int)(c->nets.h_node_off[list[i] + 1] - c->nets.h_node_off[list[i]]));
+                    cls[i] = k < 0 ? NCLS - 1 : k;
+                }
+                for (int b = 0; b < nb; b++)
+                    for (int64_t i = cnt[b]; i < cnt[b + 1]; i++) {
+                        co[(size_t)b * NCLS + cls[i] + 1]++;
+                        cbytes[(size_t)b * NCLS + cls[i]] += c->nets.h_lbytes[list[i]];
+                    }
+                for (size_t k = 0; k + 1 < co.size(); k++) co[k + 1] += co[k];
+                std::vector<int64_t> pos(co.begin(), co.end() - 1);
+                for (int b = 0; b < nb; b++)
+                    for (int64_t i = cnt[b]; i < cnt[b + 1]; i++) wl[pos[(size_t)b * NCLS + cls[i]]++] = list[i];
+            }
+            int64_t max_nodes = 0;
+            for (int b = 0; b < nb; b++) {
+                const int64_t a = co[(size_t)b * NCLS + NCLS - 1], e = co[(size_t)b * NCLS + NCLS];
+                max_nodes = std::max(max_nodes, scratch_offsets(c, wl.data() + a, e - a, off.data() + a));
+            }
+            if (max_nodes) ensure_scratch(c, max_nodes);
+            c->scratch.net_list.upload(wl.data(), n, c->stream);
+            c->scratch.scr_off.upload(off.data(), n, c->stream);
             // Costs of every track once, then per batch: DP (scattering wires,
             // vias and dirty flags), then the fused commit + cost pass over the
             // touched tracks.  The last pass leaves the costs as the plain
@@ -389,8 +426,15 @@ int igr_pattern_route(igr_ctx* c, const int* order, int64_t n, int form, igr_pas
             launch_track_fused(c, 0, 0, true);
             for (int b = 0; b < nb; b++) {
                 const int stamp = ++c->stamp;
-                launch_lshape_plain(c, c->scratch.net_list.p + cnt[b], c->scratch.scr_off.p + cnt[b],
-                                    cnt[b + 1] - cnt[b], stamp, lbytes[b]);
+                for (int k = 0; k < NCLS; k++) {
+                    const int64_t a = co[(size_t)b * NCLS + k], e = co[(size_t)b * NCLS + k + 1];
+                    if (a == e) continue;
+                    if (k < NCLS - 1)
+                        launch_lshape_wave(c, c->scratch.net_list.p + a, e - a, k, stamp, cbytes[(size_t)b * NCLS + k]);
+                    else
+                        launch_lshape_plain(c, c->scratch.net_list.p + a, c->scratch.scr_off.p + a, e - a, stamp,
+                                            cbytes[(size_t)b * NCLS + k]);
+                }
                 launch_track_fused(c, stamp, 0, b + 1 < nb);
             }
             HIP_OK(hipStreamSynchronize(c->stream));
@@ -1022,6 +1066,10 @@ int igr_set_tuning(igr_ctx* c, const igr_tuning* t) {
             throw std::invalid_argument("track_k: 0, 4, 8 or 16");
         if (t->track_k && t->track_k * 1024 < std::max(c->g.X, c->g.Y))
             throw std::invalid_argument("track_k too small for this grid's tracks");
+        if (t->presum_k != 0 && t->presum_k != 4 && t->presum_k != 8 && t->presum_k != 16)
+            throw std::invalid_argument("presum_k: 0, 4, 8 or 16");
+        if (t->presum_k && t->presum_k * 1024 < std::max(c->g.X, c->g.Y))
+            throw std::invalid_argument("presum_k too small for this grid's tracks");
         c->tune = *t;
     });
 }
@@ -1037,6 +1085,7 @@ int igr_get_tuning(igr_ctx* c, igr_tuning* t) {
         if (!t->dag_edges) t->dag_edges = 1;
         if (!t->dag_seq) t->dag_seq = DEF_DAG_SEQ;
         if (!t->track_k) t->track_k = std::max(c->g.X, c->g.Y) <= 4096 ? 4 : 8;
+        if (!t->presum_k) t->presum_k = 16;
     });
 }
 

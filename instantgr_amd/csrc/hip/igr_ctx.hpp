@@ -223,7 +223,7 @@ struct igr_ctx {
     int device = 0;
     igr_dev::DevBuf<double> w;
     igr_dev::DevBuf<int> xlen, ylen;
-    igr_dev::DevBuf<float> cap, demand, wcost, vcost;
+    igr_dev::DevBuf<float> cap, demand, wcost, vcost, exptab;
     igr_dev::DevBuf<double> presum;
     igr_dev::DevBuf<int> pre, ts;
     // Batched form: via-cut counts, rip-up differences, dirty-track flags, the
@@ -235,7 +235,7 @@ struct igr_ctx {
     igr_dev::Scratch scratch;
     igr_dev::DagDev dag;
     int stamp = 0;
-    igr_tuning tune{0, 0, 0, 0, 0, 0, 0, 0, 0};   // launch-shape overrides (0 = built-in)
+    igr_tuning tune{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};   // launch-shape overrides (0 = built-in)
     double last_dag_prep_s = 0;   // host preparation time of the last DAG pass
     double last_dag_prep_parts[4] = {0, 0, 0, 0};   // fill, batch metadata, upload, allocation
     int64_t cells() const { return (int64_t)g.L * g.X * g.Y; }
@@ -319,6 +319,12 @@ void launch_wire_commit(igr_ctx* c, int stamp);
 void launch_via_commit(igr_ctx* c, const int* d_nets, int64_t n, int stamp, int K);
 void launch_lshape_plain(igr_ctx* c, const int* d_nets, const int64_t* d_scr_off, int64_t n, int stamp,
                          double bytes = 0);
+// Batched form: the L-shape DP with a 16-lane group per net and the DP tables
+// in LDS.  Nets of one launch have node counts in one class (lshape_wave_class:
+// up to 4, 8, 16 or LSWAVE_MAX_NODES nodes; -1 above).
+constexpr int LSWAVE_MAX_NODES = 24;
+int lshape_wave_class(int nodes);
+void launch_lshape_wave(igr_ctx* c, const int* d_nets, int64_t n, int cls, int stamp, double bytes = 0);
 void launch_to_ref(igr_ctx* c, const void* src, void* dst, int elem_bytes);
 void launch_from_ref(igr_ctx* c, const void* src, void* dst, int elem_bytes);
 void launch_ripup_mark(igr_ctx* c, int th, uint8_t* d_is_of, int* d_of);
@@ -347,6 +353,8 @@ void launch_dag_batch_wave(igr_ctx* c, const DagBatch& b, int stamp);
 // Batched form: the device layout of a pass's DAGs.
 void launch_dag_layout(igr_ctx* c, const DagLayoutArgs& a, int64_t n);
 void launch_expf(const float* in, float* out, int64_t n, hipStream_t s);
+// Fills the cost kernels' exp table (EXPTAB_N floats, GridDev::exptab).
+void launch_exptab(float* t, hipStream_t s);
 // Device-to-device copy of 'bytes' repeated 'reps' times; returns GB/s
 // (bytes read + written).
 double copy_probe(int64_t bytes, int reps, hipStream_t s);

@@ -41,8 +41,13 @@ struct GridDev {
     int* rcov;
     int* rvc;
     int* dirty;
+    // igr_expf at the multiples of 1/4 from EXPTAB_LO/4 to EXPTAB_HI/4 (the
+    // cost kernels copy it to LDS; see exp_tab)
+    const float* exptab;
     int tbase[MAXL + 1];
 };
+
+constexpr int EXPTAB_LO = -348, EXPTAB_HI = 355, EXPTAB_N = EXPTAB_HI - EXPTAB_LO + 1;
 
 __host__ __device__ __forceinline__ int layer_dir(int l, int DIR) { return (l & 1) ^ DIR; }
 
@@ -75,10 +80,10 @@ __device__ __forceinline__ void emit_via(const GridDev& g, int l0, int l1, int x
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float pow2i(int k) { return __int_as_float((k + 127) << 23); }
 
-__device__ __forceinline__ float igr_expf(float x) {
-    if (x != x) return x;
-    if (x > 88.72f) return __int_as_float(0x7f800000);
-    if (x < -87.0f) return 0.0f;
+// (Branch-free: the polynomial runs on x clamped to [-87, 88.72], which
+// leaves every in-range x unchanged; the special cases are selected after.)
+__device__ __forceinline__ float igr_expf(float x0) {
+    const float x = __builtin_fminf(__builtin_fmaxf(x0, -87.0f), 88.72f);
     const float log2e = 1.44269504088896341f;
     const float ln2_hi = 0.693145751953125f;
     const float ln2_lo = 1.42860676533018704e-06f;
@@ -95,7 +100,59 @@ __device__ __forceinline__ float igr_expf(float x) {
     p = __builtin_fmaf(p, r, 1.0f);
     int k = (int)n;
     int k1 = k / 2, k2 = k - k1;
-    return (p * pow2i(k1)) * pow2i(k2);
+    float v = (p * pow2i(k1)) * pow2i(k2);
+    v = x0 > 88.72f ? __int_as_float(0x7f800000) : v;
+    v = x0 < -87.0f ? 0.0f : v;
+    return x0 != x0 ? x0 : v;
+}
+
+// igr_expf(a) for a multiple of 1/4 from the table: igr_expf is +inf above
+// 88.72 (so from 355/4 on) and 0 below -87 (so below -348/4), and the table
+// holds igr_expf of exactly the floats in between.
+// (Branch-free: a clamped index, then selects.)
+__device__ __forceinline__ float exp_tab(float a, const float* tab) {
+    const float q = a * 4.0f;   // exact: scaling by a power of two
+    const int i = (int)__builtin_fminf(__builtin_fmaxf(q, (float)EXPTAB_LO), (float)EXPTAB_HI) - EXPTAB_LO;
+    float v = tab[i];
+    v = q > (float)EXPTAB_HI ? __int_as_float(0x7f800000) : v;
+    return q < (float)EXPTAB_LO ? 0.0f : v;
+}
+
+// True when the table may serve every exponent of an edge's cost terms: an
+// integer capacity and a half-integer demand make 0.5 (d + inc - c) and
+// 1.5 (d + inc) multiples of 1/4 for inc in {0, 0.5, 1}.
+__device__ __forceinline__ bool quarter_grid(float c, float d) {
+    const float d2 = 2.0f * d;
+    return c == __builtin_rintf(c) && d2 == __builtin_rintf(d2) && __builtin_fabsf(c) < 4.0e6f &&
+           __builtin_fabsf(d) < 4.0e6f;
+}
+
+// The logistic term of src/graph.hpp:23-32 as a float, before the widening
+// to double: (double)c > 0.001 is c >= 0.001f (the float nearest 0.001 lies
+// above it), and the products 0.5 (d - c), 1.5 d and the sum d + inc are
+// exact in double, so rounding them once to float gives the float operation's
+// value.  TAB: exponent from the table (the caller has checked quarter_grid).
+// Branch-free: both exponent arguments are formed, one exp is taken.
+template <bool TAB>
+__device__ __forceinline__ float of_term(float c, float d, const float* tab) {
+    const bool with_cap = c >= 0.001f;
+    const float a = with_cap ? 0.5f * (d - c) : 1.5f * d;
+    const float e = TAB ? exp_tab(a, tab) : igr_expf(a);
+    return (with_cap || d > 0.0f) ? e : 0.0f;
+}
+
+// Cost increments of one edge for one track (a1) and half a track (ah),
+// sharing the f(d) term: inc_cost(c, d, 1.0, w) and inc_cost(c, d, 0.5, w)
+// of below, bit for bit.
+template <bool TAB>
+__device__ __forceinline__ void inc_pair(float c, float d, double w, const float* tab, double& a1, double& ah) {
+    const double f0 = (double)of_term<TAB>(c, d, tab);
+    a1 = fmin(1e12, ((double)of_term<TAB>(c, d + 1.0f, tab) - f0) * w);
+    ah = fmin(1e12, ((double)of_term<TAB>(c, d + 0.5f, tab) - f0) * w);
+}
+template <bool TAB>
+__device__ __forceinline__ double inc_one(float c, float d, float inc, double w, const float* tab) {
+    return fmin(1e12, ((double)of_term<TAB>(c, d + inc, tab) - (double)of_term<TAB>(c, d, tab)) * w);
 }
 
 // Logistic overflow cost of one edge (src/graph.hpp:23-32 with

@@ -57,3 +57,58 @@ __device__ __forceinline__ void block_sklansky(T (&s)[K], T* lds_tot /* NW entri
 }
 
 } // namespace igr_dev
+
+namespace igr_dev {
+
+// The same scan for a workgroup of nw waves known only at run time
+// (nw <= NWMAX, blockDim.x == 64 * nw).  Waves past nw hold nothing, so the
+// association order over the virtual array is the one above.
+template <typename T, int K, int NWMAX>
+__device__ __forceinline__ void block_sklansky_rt(T (&s)[K], T* lds_tot /* NWMAX entries */, int nw) {
+#pragma unroll
+    for (int d = 0; (1 << d) < K; d++) {
+#pragma unroll
+        for (int j = 0; j < K; j++)
+            if (j >> d & 1) s[j] += s[(j >> d << d) - 1];
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int D = 0; D < 6; D++) {
+        int partner = ((lane >> D) << D) - 1;
+        T v = __shfl(s[K - 1], partner < 0 ? 0 : partner, 64);
+        if (lane >> D & 1) {
+#pragma unroll
+            for (int j = 0; j < K; j++) s[j] += v;
+        }
+    }
+    if (nw > 1) {
+        if (lane == 63) lds_tot[wave] = s[K - 1];
+        __syncthreads();
+        // Wave totals after each cross-wave level, kept in registers: W[i]
+        // for i < nw.  Level b adds the last total of the preceding 2^b block.
+        T W[NWMAX];
+#pragma unroll
+        for (int i = 0; i < NWMAX; i++) W[i] = i < nw ? lds_tot[i] : T(0);
+#pragma unroll
+        for (int b = 0; (1 << b) < NWMAX; b++) {
+            if ((1 << b) >= nw) continue;   // uniform: no further level
+            const int src = ((wave >> b) << b) - 1;
+            T add = T(0);
+#pragma unroll
+            for (int i = 0; i < NWMAX; i++)
+                if (i == src) add = W[i];
+            if (wave >> b & 1) {
+#pragma unroll
+                for (int j = 0; j < K; j++) s[j] += add;
+            }
+            T Wn[NWMAX];
+#pragma unroll
+            for (int i = 0; i < NWMAX; i++) Wn[i] = (i >> b & 1) ? W[i] + W[((i >> b) << b) - 1] : W[i];
+#pragma unroll
+            for (int i = 0; i < NWMAX; i++) W[i] = Wn[i];
+        }
+        __syncthreads();
+    }
+}
+
+} // namespace igr_dev

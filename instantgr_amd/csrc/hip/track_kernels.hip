@@ -28,12 +28,12 @@ namespace {
 constexpr int SB = 256;   // block size of the scatter and fill kernels
 
 // Value of the first element of the next thread (0 past the last thread).
-template <typename T, int B>
+template <typename T>
 __device__ __forceinline__ T next_first(T first, T* sh) {
     __syncthreads();
     sh[threadIdx.x] = first;
     __syncthreads();
-    return threadIdx.x + 1 < B ? sh[threadIdx.x + 1] : T(0);
+    return threadIdx.x + 1 < blockDim.x ? sh[threadIdx.x + 1] : T(0);
 }
 // Value of the last element of the previous thread (0 for thread 0).
 template <typename T>
@@ -115,17 +115,73 @@ __device__ __forceinline__ void apply_vias(float (&dem)[K], const int (&gv)[K], 
     }
 }
 
+// Wire cost, via cost and the exclusive prefix input (the predecessor's wire
+// cost) of a thread's K cells from their demand and capacity, with pdc the
+// predecessor cell's (demand, capacity).
+template <int K, bool TAB>
+__device__ __forceinline__ void track_costs(const GridDev& g, int l, int dir, int len, int p0, const float (&dem)[K],
+                                            const float (&cap)[K], float2 pdc, const float* tab, float (&wc)[K],
+                                            float (&vc)[K], double (&s)[K]) {
+    const double w = g.w[l];
+    const int* elen = dir ? g.ylen : g.xlen;
+    // edge lengths of the cells p0 - 1 .. p0 + K - 1 (0 from the last cell on)
+    int el[K + 1];
+    el[0] = p0 >= 1 && p0 - 1 < len - 1 ? elen[p0 - 1] : 0;
+    if (K % 4 == 0 && p0 + K <= len - 1) {
+#pragma unroll
+        for (int j = 0; j < K; j += 4) {
+            const int4 q = *reinterpret_cast<const int4*>(elen + p0 + j);
+            el[j + 1] = q.x; el[j + 2] = q.y; el[j + 3] = q.z; el[j + 4] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < K; j++) el[j + 1] = p0 + j < len - 1 ? elen[p0 + j] : 0;
+    }
+    auto wire = [&](double a1, int p, int e) {
+        const float x = (float)a1;
+        const float y = (float)((double)x + g.unit_wire * (double)e);
+        return p < len - 1 ? y : x;
+    };
+    double a1p = 0.0, ahp = 0.0;
+    float wcp = 0.0f;
+    if (p0 >= 1 && p0 - 1 < len) {
+        inc_pair<TAB>(pdc.y, pdc.x, w, tab, a1p, ahp);
+        wcp = wire(a1p, p0 - 1, el[0]);
+    }
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const int p = p0 + j;
+        if (p >= len) { wc[j] = vc[j] = 0.0f; s[j] = 0.0; continue; }
+        double a1, ah;
+        inc_pair<TAB>(cap[j], dem[j], w, tab, a1, ah);
+        wc[j] = wire(a1, p, el[j + 1]);
+        float v;
+        if (p == 0) v = (float)a1;
+        else if (p == len - 1) v = (float)a1p;
+        else v = (float)(ah + ahp);
+        vc[j] = (float)((double)v + g.unit_via);
+        s[j] = p >= 1 ? (double)wcp : 0.0;   // exclusive prefix input
+        a1p = a1;
+        ahp = ah;
+        wcp = wc[j];
+    }
+}
+
 // s_c: commit stamp (0 = nothing to commit); s_r: rip-up stamp (0 = no
 // rip-up); cost: recompute costs and prefix sums (and clear the dirty flag).
 // stats[0] += cells of the tracks processed, stats[1] += those with rip-up.
-template <int K, int B>
-__global__ void __launch_bounds__(B) k_track_fused(GridDev g, int s_c, int s_r, int cost,
-                                                   unsigned long long* stats) {
-    constexpr int NW = B / 64;
-    __shared__ int sh_i[B];
-    __shared__ float2 sh_f2[B];
-    __shared__ int lds_i[NW];
-    __shared__ double lds_d[NW];
+// The workgroup is the fewest waves (at most 16) whose threads cover a track
+// at K cells each.
+template <int K>
+__global__ void __launch_bounds__(1024) k_track_fused(GridDev g, int s_c, int s_r, int cost,
+                                                      unsigned long long* stats) {
+    constexpr int NWMAX = 16;
+    const int nw = blockDim.x >> 6;
+    __shared__ int sh_i[NWMAX * 64];
+    __shared__ float2 sh_f2[NWMAX * 64];
+    __shared__ int lds_i[NWMAX];
+    __shared__ double lds_d[NWMAX];
+    __shared__ float s_exp[EXPTAB_N];
     int t = blockIdx.x;
     if (!g.dirty[t]) return;
     int l = 0;
@@ -152,7 +208,7 @@ __global__ void __launch_bounds__(B) k_track_fused(GridDev g, int s_c, int s_r, 
 #pragma unroll
         for (int j = 0; j < K; j++)
             if (inc[j] | gv[j]) clr |= 1u << j;
-        block_sklansky<int, K, NW>(inc, lds_i);
+        block_sklansky_rt<int, K, NWMAX>(inc, lds_i, nw);
 #pragma unroll
         for (int j = 0; j < K; j++) {
             const int p = threadIdx.x * K + j;
@@ -167,7 +223,7 @@ __global__ void __launch_bounds__(B) k_track_fused(GridDev g, int s_c, int s_r, 
             if (ts[j] >= s_c) gv[j] = 0;
             if (gv[j]) wr_dem |= (3u << j) >> 1;  // edges j-1 and j of this thread
         }
-        const int gn = next_first<int, B>(gv[0], sh_i);
+        const int gn = next_first<int>(gv[0], sh_i);
         if (gn) wr_dem |= 1u << (K - 1);
         apply_vias<K>(dem, gv, gn, len, 1.0f);
     }
@@ -179,8 +235,8 @@ __global__ void __launch_bounds__(B) k_track_fused(GridDev g, int s_c, int s_r, 
 #pragma unroll
         for (int j = 0; j < K; j++)
             if (dec[j] | cov[j] | gr[j]) clr |= 1u << j;
-        block_sklansky<int, K, NW>(dec, lds_i);
-        block_sklansky<int, K, NW>(cov, lds_i);
+        block_sklansky_rt<int, K, NWMAX>(dec, lds_i, nw);
+        block_sklansky_rt<int, K, NWMAX>(cov, lds_i, nw);
 #pragma unroll
         for (int j = 0; j < K; j++) {
             const int p = threadIdx.x * K + j;
@@ -192,7 +248,7 @@ __global__ void __launch_bounds__(B) k_track_fused(GridDev g, int s_c, int s_r, 
             if (ts[j] >= s_r) gr[j] = 0;
             if (gr[j]) wr_dem |= (3u << j) >> 1;
         }
-        const int gn = next_first<int, B>(gr[0], sh_i);
+        const int gn = next_first<int>(gr[0], sh_i);
         if (gn) wr_dem |= 1u << (K - 1);
         apply_vias<K>(dem, gr, gn, len, -1.0f);
     }
@@ -213,54 +269,27 @@ __global__ void __launch_bounds__(B) k_track_fused(GridDev g, int s_c, int s_r, 
     }
     if (!cost) return;
     // Costs from the new demand (src/graph.hpp:40-71).
-    // Per element the increments of one and of half a track
-    // (inc_cost(.., 1.0) and inc_cost(.., 0.5), sharing the f(d) term); the
-    // via cost takes the element's and its predecessor's, the prefix input is
-    // the predecessor's wire cost.  The predecessor of a thread's first
-    // element is recomputed from the previous thread's last demand and
-    // capacity (one exchange).
-    const double w = g.w[l];
-    const int* elen = dir ? g.ylen : g.xlen;
-    auto incs = [&](float c, float d, double& a1, double& ah) {
-        const double f0 = of_cost(c, d);
-        a1 = fmin(1e12, (of_cost(c, (float)((double)d + 1.0 + 0.0)) - f0) * w);
-        ah = fmin(1e12, (of_cost(c, (float)((double)d + 0.5 + 0.0)) - f0) * w);
-    };
-    auto wire = [&](double a1, int p) {
-        float x = (float)a1;
-        if (p < len - 1) x = (float)((double)x + g.unit_wire * (double)elen[p]);
-        return x;
-    };
+    // Per element the increments of one and of half a track (inc_pair, the
+    // shared f(d) term); the via cost takes the element's and its
+    // predecessor's, the prefix input is the predecessor's wire cost.  The
+    // predecessor of a thread's first element is recomputed from the previous
+    // thread's last demand and capacity (one exchange, whose barriers also
+    // publish the LDS exp table).  The exponents come from the table when
+    // every element of the wave allows it, else they are computed; the
+    // values are the same either way.
+    for (int i = threadIdx.x; i < EXPTAB_N; i += blockDim.x) s_exp[i] = g.exptab[i];
     const float2 pdc = prev_last(make_float2(dem[K - 1], cap[K - 1]), sh_f2);
-    double a1p = 0.0, ahp = 0.0;
-    float wcp = 0.0f;
-    if (p0 >= 1 && p0 - 1 < len) {
-        incs(pdc.y, pdc.x, a1p, ahp);
-        wcp = wire(a1p, p0 - 1);
-    }
     float wc[K], vc[K];
     double s[K];
+    bool q = quarter_grid(pdc.y, pdc.x);
 #pragma unroll
-    for (int j = 0; j < K; j++) {
-        const int p = p0 + j;
-        if (p >= len) { wc[j] = vc[j] = 0.0f; s[j] = 0.0; continue; }
-        double a1, ah;
-        incs(cap[j], dem[j], a1, ah);
-        wc[j] = wire(a1, p);
-        float v;
-        if (p == 0) v = (float)a1;
-        else if (p == len - 1) v = (float)a1p;
-        else v = (float)(ah + ahp);
-        vc[j] = (float)((double)v + g.unit_via);
-        s[j] = p >= 1 ? (double)wcp : 0.0;   // exclusive prefix input
-        a1p = a1;
-        ahp = ah;
-        wcp = wc[j];
-    }
+    for (int j = 0; j < K; j++) q = q && quarter_grid(cap[j], dem[j]);
+    if (__all(q)) track_costs<K, true>(g, l, dir, len, p0, dem, cap, pdc, s_exp, wc, vc, s);
+    else track_costs<K, false>(g, l, dir, len, p0, dem, cap, pdc, s_exp, wc, vc, s);
     store_k<K>(g.wcost, base, p0, len, v4, wc);
     store_k<K>(g.vcost, base, p0, len, v4, vc);
     // Exclusive FP64 Sklansky prefix of wcost (src/graph.hpp:405-430).
-    block_sklansky<double, K, NW>(s, lds_d);
+    block_sklansky_rt<double, K, NWMAX>(s, lds_d, nw);
     store_k<K>(g.presum, base, p0, len, v4, s);
     if (threadIdx.x == 0) g.dirty[blockIdx.x] = 0;
 }
@@ -398,25 +427,17 @@ void launch_track_fused(igr_ctx* c, int s_commit, int s_rip, bool cost) {
     unsigned long long* st = c->track_stats.p;
     hipStream_t s = c->stream;
     // Elements per thread: tuned, or 4 up to 4096-cell tracks and 8 above
-    // (4 measured fastest on 800-, 1500- and 2500-cell tracks); the block is
-    // the smallest of 256, 512, 1024 threads covering a track.
+    // (4 measured fastest on 800-, 1500- and 2500-cell tracks); the workgroup
+    // is the fewest waves covering the longest track.
     const int K = c->tune.track_k ? c->tune.track_k : (ml <= 4096 ? 4 : 8);
-    const int B = K * 256 >= ml ? 256 : K * 512 >= ml ? 512 : 1024;
-    if (K * B < ml) throw std::invalid_argument("track longer than the fused kernel's tile");
-#define IGR_TRACK_LAUNCH(KK, BB) \
-    if (K == KK && B == BB) k_track_fused<KK, BB><<<T, BB, 0, s>>>(c->g, s_commit, s_rip, cs, st)
-    if (!c->tune.track_k && ml <= 256) k_track_fused<1, 256><<<T, 256, 0, s>>>(c->g, s_commit, s_rip, cs, st);
-    else if (!c->tune.track_k && ml <= 512) k_track_fused<2, 256><<<T, 256, 0, s>>>(c->g, s_commit, s_rip, cs, st);
-    else IGR_TRACK_LAUNCH(4, 256);
-    else IGR_TRACK_LAUNCH(4, 512);
-    else IGR_TRACK_LAUNCH(4, 1024);
-    else IGR_TRACK_LAUNCH(8, 256);
-    else IGR_TRACK_LAUNCH(8, 512);
-    else IGR_TRACK_LAUNCH(8, 1024);
-    else IGR_TRACK_LAUNCH(16, 256);
-    else IGR_TRACK_LAUNCH(16, 512);
-    else IGR_TRACK_LAUNCH(16, 1024);
-#undef IGR_TRACK_LAUNCH
+    const int B = ((ml + K - 1) / K + 63) / 64 * 64;
+    if (B > 1024) throw std::invalid_argument("track longer than the fused kernel's tile");
+    switch (K) {
+    case 4: k_track_fused<4><<<T, B, 0, s>>>(c->g, s_commit, s_rip, cs, st); break;
+    case 8: k_track_fused<8><<<T, B, 0, s>>>(c->g, s_commit, s_rip, cs, st); break;
+    case 16: k_track_fused<16><<<T, B, 0, s>>>(c->g, s_commit, s_rip, cs, st); break;
+    default: throw std::invalid_argument("track_k must be 4, 8 or 16");
+    }
     HIP_OK(hipGetLastError());
 }
 

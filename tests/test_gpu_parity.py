@@ -49,7 +49,7 @@ def test_cost_update_and_presum_bitexact(design_c1):
     r.close()
 
 
-@pytest.mark.parametrize("shape", [(4, 37, 300), (5, 300, 9), (3, 1000, 7), (2, 2500, 3)])
+@pytest.mark.parametrize("shape", [(4, 37, 300), (5, 300, 9), (3, 1000, 7), (2, 2500, 3), (3, 1500, 6), (2, 12, 800)])
 def test_presum_track_lengths(shape):
     """Tracks from 7 to 2500 gcells: every chunking of the workgroup scan."""
     L, X, Y = shape
@@ -65,6 +65,37 @@ def test_presum_track_lengths(shape):
         h.set_demand(dem)
         h.update_cost()
         h.compute_presum()
+    for a, b in zip(r.costs(), o.costs()):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    # the batched form's fused track kernel over every track (workgroups of
+    # one to ten waves at these lengths)
+    r.set_demand(dem)
+    r.cost_sweep_ms(gpu.FORM_BATCHED, 1)
+    for a, b in zip(r.costs(), o.costs()):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    r.close()
+
+
+def test_cost_fractional_capacity_and_demand():
+    """Capacities and demands off the quarter grid: exponent arguments that
+    are not multiples of 1/4 are computed, not looked up; mixed in one wave
+    with ones that are."""
+    L, X, Y = 3, 300, 40
+    rng = np.random.default_rng(3)
+    cap = rng.integers(0, 7, size=L * X * Y).astype(np.float32)
+    cap[::3] *= np.float32(0.7)
+    dem = _random_demand(L * X * Y, 4)
+    dem[1::5] += np.float32(0.1)
+    r = gpu.Router(L, X, Y, 0, 1.0, 4.0, np.ones(L), np.ones(X - 1, np.int32), np.ones(Y - 1, np.int32), cap)
+    o = orc.Oracle(L, X, Y, 0, 1.0, 4.0, np.ones(L), np.ones(X - 1, np.int32), np.ones(Y - 1, np.int32), cap)
+    for h in (r, o):
+        h.set_demand(dem)
+        h.update_cost()
+        h.compute_presum()
+    for a, b in zip(r.costs(), o.costs()):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    r.set_demand(dem)
+    r.cost_sweep_ms(gpu.FORM_BATCHED, 1)
     for a, b in zip(r.costs(), o.costs()):
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
     r.close()
