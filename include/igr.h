@@ -195,6 +195,15 @@ int igr_ripup_mark(igr_ctx* ctx, int threshold, uint8_t* is_of);
 /* graph::extract_congestionView{,_xsum,_ysum} (src/graph.hpp:545-584):
  * X*Y prefix views indexed x*Y + y. */
 int igr_congestion_view(igr_ctx* ctx, float* xsum, float* ysum);
+/* No counterpart in the reference: igr_congestion_view for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same outputs; its device work
+ * uses a static scratch array of 2 * IGR_NOALLOC_MAX_GCELLS floats (64 MiB)
+ * per GPU instead of allocating, so grids of more than IGR_NOALLOC_MAX_GCELLS
+ * gcells (X*Y) are rejected with IGR_EINVAL.  All calls on one GPU share that
+ * array: calls that overlap on one GPU race. */
+#define IGR_NOALLOC_MAX_GCELLS (1 << 23)
+int igr_congestion_view_noalloc(igr_ctx* ctx, float* xsum, float* ysum);
 
 /* Device time per kernel class, measured with events on the context's stream
  * while enabled (enabling clears earlier records).  Classes: 0 cost update,

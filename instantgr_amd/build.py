@@ -20,7 +20,7 @@ INCLUDE = os.path.join(os.path.dirname(ROOT), "include")
 HOST_SRCS = ["io.cpp", "netinit.cpp", "gen.cpp", "steiner.cpp", "prep.cpp", "dag.cpp", "evaluator.cpp", "igr_host.cpp"]
 HIP_SRCS = ["grid_kernels.hip", "lshape_kernels.hip", "sched_kernels.hip", "sched_batched.hip", "dag_kernels.hip", "dag_layout.hip", "track_kernels.hip",
             "igr_ctx.hip", "serve.hip", "eval_kernels.hip",
-            "pipeline.hip"]
+            "pipeline.hip", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

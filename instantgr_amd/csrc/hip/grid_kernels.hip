@@ -539,16 +539,25 @@ void launch_ripup_mark(igr_ctx* c, int th, uint8_t* d_is_of, int* d_of) {
     HIP_OK(hipGetLastError());
 }
 
-void launch_congestion(igr_ctx* c, float* d_xsum, float* d_ysum) {
+void launch_congestion_map(igr_ctx* c, float* d_v) {
     const GridDev& g = c->g;
-    DevBuf<float> v;
-    v.alloc(g.XYc);
-    k_congestion<<<blocks(g.XYc), TPB, 0, c->stream>>>(g, v.p);
+    k_congestion<<<blocks(g.XYc), TPB, 0, c->stream>>>(g, d_v);
     HIP_OK(hipGetLastError());
-    // along x for every y: elements x*Y + y, lines y
-    DISPATCH_K(g.X, k_view_scan, g.Y, c->stream, v.p, d_xsum, g.X, (int64_t)g.Y, (int64_t)1);
-    // along y for every x: elements x*Y + y, lines x
-    DISPATCH_K(g.Y, k_view_scan, g.X, c->stream, v.p, d_ysum, g.Y, (int64_t)1, (int64_t)g.Y);
+}
+
+void launch_view_scan(igr_ctx* c, const float* d_v, float* d_out, int axis) {
+    const GridDev& g = c->g;
+    if (axis == 0)   // along x for every y: elements x*Y + y, lines y
+        DISPATCH_K(g.X, k_view_scan, g.Y, c->stream, d_v, d_out, g.X, (int64_t)g.Y, (int64_t)1);
+    else             // along y for every x: elements x*Y + y, lines x
+        DISPATCH_K(g.Y, k_view_scan, g.X, c->stream, d_v, d_out, g.Y, (int64_t)1, (int64_t)g.Y);
+}
+
+void launch_congestion(igr_ctx* c, float* d_xsum, float* d_ysum) {
+    c->scratch.view.alloc(c->g.XYc);
+    launch_congestion_map(c, c->scratch.view.p);
+    launch_view_scan(c, c->scratch.view.p, d_xsum, 0);
+    launch_view_scan(c, c->scratch.view.p, d_ysum, 1);
     HIP_OK(hipStreamSynchronize(c->stream));
 }
 

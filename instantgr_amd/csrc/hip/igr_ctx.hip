@@ -71,6 +71,7 @@ int igr_create(const igr_grid_desc* d, igr_ctx** out) {
         HIP_OK(hipGetDeviceCount(&ndev));
         if (ndev < 1) throw std::runtime_error("no GPU visible");
         HIP_OK(hipGetDevice(&c->device));
+        noalloc_init(c->device);
         HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIP_OK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
         GridDev& g = c->g;

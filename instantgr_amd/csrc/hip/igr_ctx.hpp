@@ -177,7 +177,7 @@ struct Scratch {
     DevBuf<int64_t> scr_off;
     DevBuf<int> of;
     DevBuf<uint8_t> is_of;
-    DevBuf<float> xsum, ysum;
+    DevBuf<float> xsum, ysum, view;
 };
 
 // Per-kernel-class device time, measured with events on the context stream
@@ -329,6 +329,14 @@ void launch_to_ref(igr_ctx* c, const void* src, void* dst, int elem_bytes);
 void launch_from_ref(igr_ctx* c, const void* src, void* dst, int elem_bytes);
 void launch_ripup_mark(igr_ctx* c, int th, uint8_t* d_is_of, int* d_of);
 void launch_congestion(igr_ctx* c, float* d_xsum, float* d_ysum);
+// Its parts, on caller buffers: the per-gcell view (X*Y floats), and one
+// prefix view of it (axis 0 along x, 1 along y).  No allocation, no sync.
+void launch_congestion_map(igr_ctx* c, float* d_v);
+void launch_view_scan(igr_ctx* c, const float* d_v, float* d_out, int axis);
+// Static scratch of the no-allocation entry points: looked up once per
+// device (call with that device current); the error of the lookup, if any,
+// is kept for those entry points to report.
+void noalloc_init(int device);
 // Scheduler; with d_pt_off/d_pt given, the points of order[i] are list entry i
 // of those arrays instead of the pattern footprint.
 int run_schedule(igr_ctx* c, const int* h_order, int64_t n, int mode, int* h_batch_of,

@@ -114,6 +114,7 @@ SIGNATURES = {
     "igr_routes_capacity": (C.c_int64, [C.c_void_p]),
     "igr_ripup_mark": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "igr_congestion_view": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "igr_congestion_view_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "igr_stamp": (C.c_int, [C.c_void_p]),
     "igr_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "igr_kernel_times": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -353,6 +354,14 @@ class Router:
         xs = np.empty(self.X * self.Y, np.float32)
         ys = np.empty(self.X * self.Y, np.float32)
         _check(lib().igr_congestion_view(self.h, xs.ctypes.data, ys.ctypes.data))
+        return xs, ys
+
+    def congestion_view_noalloc(self):
+        """congestion_view through the entry point that allocates no device
+        memory (igr_congestion_view_noalloc)."""
+        xs = np.empty(self.X * self.Y, np.float32)
+        ys = np.empty(self.X * self.Y, np.float32)
+        _check(lib().igr_congestion_view_noalloc(self.h, xs.ctypes.data, ys.ctypes.data))
         return xs, ys
 
     def set_profiling(self, on=True):
