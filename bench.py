@@ -37,11 +37,20 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=1, help="benchmark configuration index (1 = 800x800, 500k nets)")
+    ap.add_argument("--config", type=int, default=2,
+                    help="benchmark configuration index (0 = 200x200 20k nets, 1 = 800x800 500k nets, "
+                         "2 = 1500x1500 3M nets with heavy nets and hotspots, 3 = 12 layers 2500x2500 8M nets)")
+    ap.add_argument("--budget", type=float, default=380.0,
+                    help="wall seconds (from process start) within which warmup and timed steps of the headline "
+                         "configuration start; at least one timed step always runs")
+    ap.add_argument("--compare-config", type=int, default=1,
+                    help="also run both forms one step each on this configuration in the same process "
+                         "(-1: off)")
     ap.add_argument("--nets", type=int, default=0, help="override the net count (testing)")
     ap.add_argument("--rrr", type=int, default=3)
     ap.add_argument("--threads", type=int, default=16)
-    ap.add_argument("--forms", default="plain,batched")
+    ap.add_argument("--forms", default="batched",
+                    help="forms timed on the headline configuration (plain,batched: both, with route identity)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true", help="skip the untimed kernel-timing step")
     ap.add_argument("--serve", metavar="RATES",
@@ -141,17 +150,16 @@ def roofline(kt, kb):
 
 def cpu_baseline():
     """The CPU oracle (sequential restatement of the reference logic, one
-    thread) routing a bounded sample of the same workload: the generator's
-    configuration-1 distributions at the same net density on a 176x176 grid
-    (24,200 nets), tree build + pattern pass + 3 rip-up-and-reroute passes."""
+    thread) routing configuration 0 in full (10 metal layers, 200x200 gcells,
+    20k nets): tree build + pattern pass + 3 rip-up-and-reroute passes."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
     from instantgr_amd.host import Design
-    nets = 24200
-    d = Design.custom(layers=10, xsize=176, ysize=176, nets=nets, pin_cap=64, seed=2)
+    d = Design.preset(0)
     t0 = time.perf_counter()
     d.build(1)
+    nets = d.info.nets
     o = orc.Oracle.from_design(d)
     o.set_nets_from_design(d)
     o.pattern_route(d.array("order"))
@@ -166,18 +174,10 @@ def cpu_baseline():
         dg = d.build_dags(sel, xs, ys, 0, 1)
         orc.dag_pass(o, sel, dg, 0)
     dt = time.perf_counter() - t0
-    cpu = ""
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
     return {"value": nets / dt, "unit": "nets/s", "cores": 1, "kind": "port",
-            "sample": "10 layers, 176x176 gcells, 24200 nets (configuration-1 generator and net density), "
-                      "tree build + pattern pass + 3 RRR passes, one thread; %.1f s on %s (%d host cores)"
-                      % (dt, cpu or "unknown CPU", os.cpu_count() or 0)}
+            "sample": "configuration 0 in full (10 metal layers, 200x200 gcells, %d nets), tree build + pattern pass "
+                      "+ 3 RRR passes, one thread: %.1f s on %s (%d host cores)"
+                      % (nets, dt, _cpu_model(), os.cpu_count() or 0)}
 
 
 def _find(root, suffix):
@@ -269,7 +269,7 @@ def summarize_lds(root):
         print("%-24s %10d %18.0f %18.0f %12s" % (k, n, c, t, "%.4f" % (c / t) if t > 0 else "no LDS"))
 
 
-def cost_prefix_sweeps(design, reps=20):
+def cost_prefix_sweeps(design, reps=20, track_k=0):
     """Full-grid cost update + prefix sums in both forms, on the benchmark grid
     and on the largest grid the generator defines (12 metal layers,
     2500x2500).  GB/s over the algorithmic bytes per cell: 28 for the plain
@@ -289,6 +289,8 @@ def cost_prefix_sweeps(design, reps=20):
             cap = rng.integers(2, 7, size=L * X * Y).astype(np.float32)
             r = gpu.Router(L, X, Y, 1, 1.0, 4.0, np.ones(L), np.ones(X - 1, np.int32), np.ones(Y - 1, np.int32), cap)
         cells = r.n
+        if track_k:
+            r.set_tuning(track_k=track_k)
         row = {"grid": name, "cells": cells}
         for form, key, per in ((gpu.FORM_PLAIN, "plain", 28.0), (gpu.FORM_BATCHED, "fused", 24.0)):
             ms = r.cost_sweep_ms(form, reps)
@@ -339,6 +341,156 @@ def serve_bench(a):
                       "rates": lines}), flush=True)
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def run_form(design, form, name, a, tuning, budget_s, t_origin):
+    """Warmup steps (the first of them with kernel timing on: the kernel
+    times and algorithmic bytes of the roofline come from it, untimed), then
+    timed steps, each a full pipeline on a fresh context.  A step starts only
+    while the wall time since t_origin plus one step's time stays inside
+    budget_s (at least one timed step runs).  Returns the measurements and the
+    last timed step's router (routes, score)."""
+    from instantgr_amd import gpu
+    tun = tuning if name == "batched" else None
+    est = None
+    warm = 0
+    kt = kb = None
+    for w in range(a.warmup):
+        if est is not None and time.perf_counter() - t_origin + 2 * est > budget_s:
+            break
+        prof = w == 0 and not a.no_profile
+        r, st = one_step(design, form, a.rrr, a.threads, profile=prof, tuning=tun)
+        if prof:
+            kt, kb = r.kernel_times(), r.kernel_bytes()
+            prof_routes = r.routes_flat()
+        r.close()
+        est = st["step_s"]
+        warm += 1
+        _progress("%s warmup %d: %.3f s%s" % (name, w, st["step_s"], " (kernel timing on)" if prof else ""))
+    times, last = [], None
+    for k in range(a.steps):
+        if k > 0 and est is not None and time.perf_counter() - t_origin + est > budget_s:
+            break
+        r, st = one_step(design, form, a.rrr, a.threads, tuning=tun)
+        times.append(st["step_s"])
+        est = st["step_s"]
+        _progress("%s step %d: %.3f s" % (name, k, st["step_s"]))
+        if last is not None:
+            last[0].close()
+        last = (r, st)
+    r, st = last
+    routes = r.routes_flat()
+    if kt is None and not a.no_profile:
+        # no warmup ran: one extra, untimed step with kernel timing
+        r2, _ = one_step(design, form, a.rrr, a.threads, profile=True, tuning=tun)
+        kt, kb = r2.kernel_times(), r2.kernel_bytes()
+        prof_routes = r2.routes_flat()
+        r2.close()
+    if kt is not None and not gpu.routes_equal(prof_routes, routes):
+        raise RuntimeError("the kernel-timing step routed differently")
+    t_ev = time.perf_counter()
+    gscore = r.evaluate(design)
+    gscore["eval_s"] = time.perf_counter() - t_ev
+    _progress("%s GPU evaluator: %s" % (name, json.dumps(gscore)))
+    print("[bench] %s: %s" % (name, json.dumps({"step_s": times, "warmup": warm, "pattern_batches": st["pattern_batches"],
+                                               "ripped": st["ripped"], "rrr_batches": st["rrr_batches"],
+                                               "tree_s": st["tree_s"], "pattern_s": st["pattern_s"],
+                                               "pattern_schedule_s": st["pattern_schedule_s"],
+                                               "rrr_dag_build_s": st["rrr_dag_build_s"],
+                                               "rrr_route_s": st["rrr_route_s"],
+                                               "rrr_schedule_s": st["rrr_schedule_s"],
+                                               "kernel_ms": {k: round(v[0], 1) for k, v in (kt or {}).items() if v[1]}})),
+          file=sys.stderr, flush=True)
+    r.close()
+    return {"step_s": times, "warmup": warm, "stats": st, "kernels": kt or {}, "bytes": kb or {}, "gpu_score": gscore,
+            "routes": routes}
+
+
+def host_scores(design, results):
+    """Each form's routes written as the route file and scored by the host
+    evaluator, checked against the GPU evaluator's score."""
+    scores = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for f, res in results.items():
+            p = os.path.join(tmp, f + ".out")
+            t_h = time.perf_counter()
+            design.write_routes(p, *res["routes"])
+            h = design.evaluate(p)
+            h["write_eval_s"] = time.perf_counter() - t_h
+            g = res["gpu_score"]
+            if h["opens"] != g["opens"] or abs(h["total"] - g["total"]) > 1e-9 * abs(h["total"]):
+                raise RuntimeError("GPU and host evaluators disagree: %s vs %s" % (g, h))
+            scores[f] = dict(g, host=h)
+    return scores
+
+
+def same_routes(results):
+    from instantgr_amd import gpu
+    if len(results) != 2:
+        return None
+    (fa, ra), (fb, rb) = results.items()
+    same = gpu.routes_equal(ra["routes"], rb["routes"])
+    ga, gb = ra["gpu_score"], rb["gpu_score"]
+    if same and (ga["opens"] != gb["opens"] or abs(ga["total"] - gb["total"]) > 1e-9 * abs(gb["total"])):
+        raise RuntimeError("identical routes scored differently")
+    return same
+
+
+def workload_name(cfg, info, rrr):
+    return ("configuration %d: %d metal layers, %dx%d gcells, %d nets; tree build + pattern pass + %d "
+            "rip-up-and-reroute passes" % (cfg, info.metal_layers, info.X, info.Y, info.nets, rrr))
+
+
+def comparison_run(cfg, a, t_origin):
+    """Both forms on another configuration in the same process, one step each:
+    plain vs batched time, identical routes, equal GPU and host scores."""
+    from instantgr_amd import gpu
+    from instantgr_amd.host import Design
+    import argparse as _ap
+    design = Design.preset(cfg)
+    b = _ap.Namespace(**vars(a))
+    b.warmup, b.steps, b.no_profile = 0, 1, True
+    res = {}
+    for name, form in (("plain", gpu.FORM_PLAIN), ("batched", gpu.FORM_BATCHED)):
+        res[name] = run_form(design, form, name, b, None, float("inf"), t_origin)
+    same = same_routes(res)
+    sc = host_scores(design, res)
+    info = design.info
+    out = {"workload": workload_name(cfg, info, a.rrr), "subnets": int(info.subnets), "routes_identical": same,
+           "plain_ms_per_step": 1e3 * res["plain"]["step_s"][0], "batched_ms_per_step": 1e3 * res["batched"]["step_s"][0],
+           "speedup_batched_vs_plain": res["plain"]["step_s"][0] / res["batched"]["step_s"][0],
+           "score": {f: sc[f] for f in sc}}
+    _progress("comparison on configuration %d: %s" % (cfg, json.dumps({k: v for k, v in out.items() if k != "score"})))
+    return out
+
+
+def committed_plain_run():
+    """The plain form's full step on the headline configuration from the
+    committed summary of a run with both forms in one process (lines
+    'plain_ms_per_step V', 'batched_ms_per_step V', 'routes_identical V')."""
+    path = os.path.join(PROFILES, "bench_config2_both_forms.txt")
+    if not os.path.exists(path):
+        return None
+    out = {"source": "profiles/bench_config2_both_forms.txt"}
+    for line in open(path):
+        f = line.split()
+        if len(f) == 2 and f[0] in ("plain_ms_per_step", "batched_ms_per_step"):
+            out[f[0]] = float(f[1])
+        elif len(f) == 2 and f[0] == "routes_identical":
+            out[f[0]] = f[1] == "true"
+    if "plain_ms_per_step" in out and "batched_ms_per_step" in out:
+        out["speedup_batched_vs_plain"] = out["plain_ms_per_step"] / out["batched_ms_per_step"]
+    return out
+
+
 def main():
     a = parse()
     if a.summarize_stats:
@@ -355,6 +507,7 @@ def main():
               "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
         sys.exit(2)
     os.environ.setdefault("IGR_PROGRESS", "1")
+    t_origin = time.perf_counter()
     from instantgr_amd import gpu
     from instantgr_amd.host import Design
     if a.serve:
@@ -375,128 +528,85 @@ def main():
     forms = [f.strip() for f in a.forms.split(",") if f.strip()]
     form_id = {"plain": gpu.FORM_PLAIN, "batched": gpu.FORM_BATCHED}
     results = {}
-    routes = {}
     for f in forms:
-        for w in range(a.warmup):
-            r, _st = one_step(design, form_id[f], a.rrr, a.threads, tuning=tuning if f == "batched" else None)
-            r.close()
-            _progress("%s warmup %d: %.3f s" % (f, w, _st["step_s"]))
-        times = []
-        last = None
-        for k in range(a.steps):
-            r, st = one_step(design, form_id[f], a.rrr, a.threads, tuning=tuning if f == "batched" else None)
-            times.append(st["step_s"])
-            _progress("%s step %d: %.3f s" % (f, k, st["step_s"]))
-            if last is not None:
-                last[0].close()
-            last = (r, st)
-        r, st = last
-        routes[f] = r.routes_flat()
-        # Score of the routes on the device (GPU evaluator), before the
-        # context goes.
-        t_ev = time.perf_counter()
-        gscore = r.evaluate(design)
-        gscore["eval_s"] = time.perf_counter() - t_ev
-        _progress("%s GPU evaluator: %s" % (f, json.dumps(gscore)))
-        if not a.no_profile:
-            # Kernel timing (events on the stream, no graph) in one extra,
-            # untimed step.
-            r.close()
-            r, pst = one_step(design, form_id[f], a.rrr, a.threads, profile=True,
-                              tuning=tuning if f == "batched" else None)
-            _progress("%s profiled step: %.3f s" % (f, pst["step_s"]))
-            if not gpu.routes_equal(r.routes_flat(), routes[f]):
-                raise RuntimeError("profiled step routed differently")
-        kt = r.kernel_times()
-        kb = r.kernel_bytes()
-        results[f] = {"step_s": times, "stats": st, "kernels": kt, "bytes": kb, "gpu_score": gscore}
-        print("[bench] %s: %s" % (f, json.dumps({"step_s": times, "pattern_batches": st["pattern_batches"],
-                                                   "ripped": st["ripped"], "rrr_batches": st["rrr_batches"],
-                                                   "tree_s": st["tree_s"],
-                                                   "pattern_s": st["pattern_s"],
-                                                   "rrr_dag_build_s": st["rrr_dag_build_s"],
-                                                   "rrr_route_s": st["rrr_route_s"],
-                                                   "rrr_schedule_s": st["rrr_schedule_s"],
-                                                   "rrr_prep_s": st["rrr_prep_s"],
-                                                   "rrr_prep_parts": st["rrr_prep_parts"],
-                                                   "pattern_schedule_s": st["pattern_schedule_s"],
-                                                   "kernel_ms": {k: round(v[0], 1) for k, v in kt.items() if v[1]}})),
-              file=sys.stderr, flush=True)
-        r.close()
-
+        results[f] = run_form(design, form_id[f], f, a, tuning, a.budget, t_origin)
     info = design.info
     nets_in = info.nets
-    # Identical routes and score across forms.  The score is the GPU
-    # evaluator's; the host evaluator scores the written route file beside it
-    # (the check that the file the command line writes scores the same),
-    # skipped on designs above configuration 1 unless asked for.
-    same = None
+    same = same_routes(results)
+    # Host evaluator on the written route file beside the GPU evaluator (the
+    # check that the file the command line writes scores the same); on the
+    # large designs only when asked (two 11M-subnet files take minutes).
     scores = {f: results[f]["gpu_score"] for f in forms}
     if a.host_score or a.config <= 1:
-        with tempfile.TemporaryDirectory() as tmp:
-            for f in forms:
-                off, data = routes[f]
-                p = os.path.join(tmp, f + ".out")
-                t_h = time.perf_counter()
-                design.write_routes(p, off, data)
-                h = design.evaluate(p)
-                h["write_eval_s"] = time.perf_counter() - t_h
-                g = scores[f]
-                if h["opens"] != g["opens"] or abs(h["total"] - g["total"]) > 1e-9 * abs(h["total"]):
-                    raise RuntimeError("GPU and host evaluators disagree: %s vs %s" % (g, h))
-                scores[f] = dict(g, host=h)
-    if len(forms) == 2:
-        same = gpu.routes_equal(routes[forms[0]], routes[forms[1]])
-        g0, g1 = (results[f]["gpu_score"] for f in forms)
-        if same and (g0["opens"] != g1["opens"] or abs(g0["total"] - g1["total"]) > 1e-9 * abs(g1["total"])):
-            raise RuntimeError("identical routes scored differently")
-    sweeps = cost_prefix_sweeps(design)
+        scores = host_scores(design, results)
+    head = "batched" if "batched" in results else forms[-1]
+    # Same-run plain vs batched on the 800x800 design, and the plain form's
+    # full step on this design from the committed both-forms run.
+    comparison = None
+    if a.compare_config >= 0 and a.compare_config != a.config and not ("plain" in results and "batched" in results):
+        comparison = comparison_run(a.compare_config, a, t_origin)
+    sweeps = cost_prefix_sweeps(design, track_k=(tuning or {}).get("track_k", 0))
     cli = None
     if a.cli:
-        # The drop-in command line: parse .cap/.net, trees, route, write the
-        # route file; wall time from process start to exit, I/O included.
-        import subprocess
-        from instantgr_amd import build as igr_build
-        exe = os.path.join(os.path.dirname(igr_build.__file__), "lib", "igr_route")
-        with tempfile.TemporaryDirectory() as tmp:
-            cap, net, outp = (os.path.join(tmp, n) for n in ("d.cap", "d.net", "d.out"))
-            design.save(cap, net)
-            t_cli = time.perf_counter()
-            proc = subprocess.run([exe, "-cap", cap, "-net", net, "-out", outp, "-form", "batched",
-                                   "-rrr", str(a.rrr), "-threads", str(a.threads)], capture_output=True, text=True)
-            wall = time.perf_counter() - t_cli
-            if proc.returncode != 0:
-                raise RuntimeError("igr_route failed: " + proc.stderr)
-            cli = {"wall_s": wall, "files_mb": (os.path.getsize(cap) + os.path.getsize(net) +
-                                                os.path.getsize(outp)) / 1e6,
-                   "score": design.evaluate(outp)["total"], "stages": proc.stdout.strip()}
-            _progress("command line: %.3f s (%s)" % (wall, cli["stages"]))
-    head = "batched" if "batched" in results else forms[-1]
+        cli = cli_run(design, a)
     step_ms = 1e3 * sum(results[head]["step_s"]) / len(results[head]["step_s"])
     value = nets_in / (step_ms * 1e-3)
+    steps_run = len(results[head]["step_s"])
     out = {
         "metric": "nets routed/sec end-to-end (batched vs plain, same GPU) at identical eval score",
-        "value": value, "unit": "nets/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "value": value, "unit": "nets/s", "n_gpus": 1, "steps": steps_run, "warmup": results[head]["warmup"],
         "ms_per_step": step_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "configuration %d: %d metal layers, %dx%d gcells, %d nets; tree build + pattern "
-                               "pass + %d rip-up-and-reroute passes" % (a.config, info.metal_layers, info.X, info.Y,
-                                                                        nets_in, a.rrr),
-                   "subnets": int(info.subnets), "form": head, "tuning": tuning},
+        "config": {"workload": workload_name(a.config, info, a.rrr), "subnets": int(info.subnets), "form": head,
+                   "tuning": tuning},
         "forms": {f: {"nets_per_s": nets_in / (sum(v["step_s"]) / len(v["step_s"])),
                       "ms_per_step": 1e3 * sum(v["step_s"]) / len(v["step_s"]),
                       "score": scores[f]} for f, v in results.items()},
         "routes_identical": same,
-        "cli": cli,
-        "cost_prefix_sweep": sweeps,
-        "roofline": roofline(results[head]["kernels"], results[head]["bytes"]),
     }
-    if copy_gbs is not None:
-        out["roofline"]["copy_probe_GBs"] = copy_gbs
+    if steps_run != a.steps or results[head]["warmup"] != a.warmup:
+        out["steps_requested"] = a.steps
+        out["warmup_requested"] = a.warmup
+        out["note"] = ("one step is the whole pipeline on this design (%.0f s); %d warmup and %d timed steps fit the "
+                       "%.0f s budget (--budget), of the %d and %d asked for"
+                       % (step_ms / 1e3, results[head]["warmup"], steps_run, a.budget, a.warmup, a.steps))
     if "plain" in results and "batched" in results:
         out["speedup_batched_vs_plain"] = out["forms"]["plain"]["ms_per_step"] / out["forms"]["batched"]["ms_per_step"]
+    else:
+        committed = committed_plain_run() if a.config == 2 else None
+        if committed:
+            out["plain_on_this_design"] = committed
+    if comparison:
+        out["same_run_comparison"] = comparison
+    out["cli"] = cli
+    out["cost_prefix_sweep"] = sweeps
+    out["roofline"] = roofline(results[head]["kernels"], results[head]["bytes"])
+    if copy_gbs is not None:
+        out["roofline"]["copy_probe_GBs"] = copy_gbs
     out["cpu_baseline"] = None if a.no_cpu else cpu_baseline()
     print(json.dumps(out), flush=True)
+
+
+def cli_run(design, a):
+    """The drop-in command line: parse .cap/.net, trees, route, write the
+    route file; wall time from process start to exit, I/O included."""
+    import subprocess
+    from instantgr_amd import build as igr_build
+    exe = os.path.join(os.path.dirname(igr_build.__file__), "lib", "igr_route")
+    with tempfile.TemporaryDirectory() as tmp:
+        cap, net, outp = (os.path.join(tmp, n) for n in ("d.cap", "d.net", "d.out"))
+        design.save(cap, net)
+        t_cli = time.perf_counter()
+        proc = subprocess.run([exe, "-cap", cap, "-net", net, "-out", outp, "-form", "batched",
+                               "-rrr", str(a.rrr), "-threads", str(a.threads)], capture_output=True, text=True)
+        wall = time.perf_counter() - t_cli
+        if proc.returncode != 0:
+            raise RuntimeError("igr_route failed: " + proc.stderr)
+        cli = {"wall_s": wall, "files_mb": (os.path.getsize(cap) + os.path.getsize(net) +
+                                            os.path.getsize(outp)) / 1e6,
+               "score": design.evaluate(outp)["total"], "stages": proc.stdout.strip()}
+        _progress("command line: %.3f s (%s)" % (wall, cli["stages"]))
+    return cli
 
 
 if __name__ == "__main__":

@@ -124,3 +124,37 @@ def test_request_validation():
         res = srv.wait(srv.submit(np.array([1, 2, 3], np.int32)))
         assert res["nets"] == 3 and res["latency_s"] >= 0
     r.close()
+
+
+@pytest.mark.gpu
+def test_requests_on_the_800x800_design_match_the_oracle():
+    """Serving on the routed 800x800 design (500k nets): each reroute request,
+    served alone in its tick, leaves the demand, the timestamps and every
+    route the oracle's rip-up-and-reroute pass gives when replayed from the
+    same state (view, DAGs of the request's subnets by decreasing HPWL, one
+    DAG pass: src/Lshape_route_detour.hpp:726-1112)."""
+    import orc
+    from instantgr_amd import gpu
+    from instantgr_amd.host import Design
+    d = Design.preset(1).build(16)
+    r = gpu.Router.from_design(d)
+    r.run(d, form=gpu.FORM_BATCHED, rrr=3, threads=16)
+    o = orc.Oracle.from_design(d)
+    o.set_nets_from_design(d)
+    o.set_state(r.demand(), r.timestamp(), r.stamp, r.routes_flat())
+    hp = d.array("hpwl")
+    reqs = serve.make_requests(d, 4, rate=10.0, window=50, kmax=1500, seed=21)
+    assert any(len(q) > 100 for _, q in reqs)
+    for _, q in reqs:
+        with gpu.Server(r, d, threads=16) as srv:
+            res = srv.wait(srv.submit(q))
+        assert res["tick_nets"] == len(q)
+        xs, ys = o.congestion_view()
+        nets = q[np.argsort(-hp[q], kind="stable")].astype(np.int32)
+        orc.dag_pass(o, nets, d.build_dags(nets, xs, ys, 0, 16), 0)
+        assert np.array_equal(r.demand().view(np.uint32), o.demand().view(np.uint32))
+        assert np.array_equal(r.timestamp(), o.timestamp())
+        for n in q:
+            assert np.array_equal(r.route(int(n)), o.route(int(n))), int(n)
+    assert gpu.routes_equal(r.routes_flat(), o.routes_flat())
+    r.close()
