@@ -62,6 +62,9 @@ def parse():
                          'profiler runs use)')
     ap.add_argument("--cli", action="store_true",
                     help="also time the command-line stage (files in, route file out) on the same design")
+    ap.add_argument("--tuning-file", default=os.path.join(ROOT, "instantgr_amd", "tuning.json"),
+                    help="launch shapes chosen by the autotuner for this design (autotune.save format), used "
+                         "when present and neither --tuning nor --autotune-nets is given")
     ap.add_argument("--autotune-nets", type=int, default=0,
                     help="before timing, tune the batched launch shapes on a sample design of this many nets "
                          "from the same generator (untimed)")
@@ -122,6 +125,9 @@ def roofline(kt, kb):
     those bytes; every class is listed beside it.  The scheduler class is a
     sequence of small round kernels timed as one group, with no byte model:
     when it holds the most time it is named in 'largest_class'."""
+    if not kt or all(v[0] <= 0 for v in kt.values()):
+        return {"kernel": None, "bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "achieved": None, "frac": None,
+                "traffic": None, "note": "kernel timing off (--no-profile)"}
     priced = {k: v for k, v in kt.items() if kb.get(k, 0) > 0 and v[0] > 0}
     name, (ms, launches) = max((priced or kt).items(), key=lambda kv: kv[1][0])
     out = {"kernel": name, "symbol": KERNEL_SYMBOL.get(name), "total_ms": ms, "launches": launches,
@@ -491,6 +497,11 @@ def committed_plain_run():
     return out
 
 
+def tuning_key(config, nets=0):
+    """Key of a benchmark design in the autotuner's cache file."""
+    return "config%d" % config + ("_nets%d" % nets if nets else "")
+
+
 def main():
     a = parse()
     if a.summarize_stats:
@@ -518,6 +529,14 @@ def main():
     design = Design.preset(a.config, nets=a.nets)
     _progress("design generated in %.1f s" % (time.perf_counter() - t_gen))
     tuning = json.loads(a.tuning) if a.tuning else None
+    tuning_source = "command line" if tuning else None
+    if tuning is None and not a.autotune_nets and a.tuning_file and os.path.exists(a.tuning_file):
+        with open(a.tuning_file) as f:
+            cached = json.load(f).get(tuning_key(a.config, a.nets))
+        if cached:
+            tuning = cached["tuning"]
+            tuning_source = "autotuner (%s)" % cached.get("sample", "cached")
+            _progress("launch shapes from the autotuner's cache: %s" % json.dumps(tuning))
     if a.autotune_nets:
         from instantgr_amd import autotune
         t_at = time.perf_counter()
@@ -525,6 +544,7 @@ def main():
         tuning = autotune.tune(sample, rrr=1, threads=a.threads, log=_progress)["tuning"]
         _progress("autotuned on %d nets in %.1f s: %s" % (a.autotune_nets, time.perf_counter() - t_at,
                                                         json.dumps(tuning)))
+        tuning_source = "autotuner (%d-net sample, this run)" % a.autotune_nets
     forms = [f.strip() for f in a.forms.split(",") if f.strip()]
     form_id = {"plain": gpu.FORM_PLAIN, "batched": gpu.FORM_BATCHED}
     results = {}
@@ -558,7 +578,7 @@ def main():
         "ms_per_step": step_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": workload_name(a.config, info, a.rrr), "subnets": int(info.subnets), "form": head,
-                   "tuning": tuning},
+                   "tuning": tuning, "tuning_source": tuning_source},
         "forms": {f: {"nets_per_s": nets_in / (sum(v["step_s"]) / len(v["step_s"])),
                       "ms_per_step": 1e3 * sum(v["step_s"]) / len(v["step_s"]),
                       "score": scores[f]} for f, v in results.items()},

@@ -236,6 +236,9 @@ typedef struct igr_tuning {
                            active nets), 2 release-driven (work follows the released cells); built-in
                            choice 2 from 1.5M scheduled nets */
     int presum_k;       /* plain form's prefix-sum kernel: cells per thread, 4, 8 or 16 */
+    int dag_grid;       /* DAG kernel: persistent workgroups pulling a batch's nets from its work queue */
+    int lshape_groups;  /* pattern DP: nets (16-lane groups) per workgroup, 1, 2 or 4; the workgroup's LDS
+                           slab holds their layer-range tables (0: 4, fewer where the tables need more room) */
 } igr_tuning;
 int igr_set_tuning(igr_ctx* ctx, const igr_tuning* t);
 /* The values in force (built-in choices resolved, per-pass choices as 0). */

@@ -13,6 +13,14 @@ results (include/igr.h, igr_tuning):
   dag_seq      node count from which a DAG level runs a node per lane instead of 16 lanes per node
   sched_kind   scheduler rounds: active-list (work follows the active nets) or release-driven
                (work follows the released cells)
+  dag_grid     persistent workgroups of the DAG kernel pulling nets from the batch's work queue
+  lshape_groups  nets (16-lane groups) per workgroup of the pattern DP, which sets the LDS slab a
+               workgroup holds (their layer-range tables)
+
+In the terms of a tile shape: nets per workgroup (lshape_groups, dag_grid),
+LDS slab size (lshape_groups times the node tables of the node-count class),
+and layers per wave (dag_seq: 16 lanes per node, one layer per lane, four
+nodes per wave; or a node per lane holding all its layers, 64 per wave).
 
 `tune` measures them on a design (normally a reduced sample of the production
 design from the same generator) by coordinate descent: each field in turn takes
@@ -39,12 +47,17 @@ CANDIDATES = {
     "dag_edges": [1, 2],
     "dag_seq": [4, 8, 16, 32, 1 << 30],
     "sched_kind": [1, 2],
+    "dag_grid": [1024, 2048, 4096],
+    "lshape_groups": [1, 2, 4],
 }
+
+# Fields of the plain form only (the tuner measures the batched form).
+PLAIN_ONLY = {"presum_k"}
 
 # Which measured stage each field affects.
 STAGE = {"sched_group": "schedule", "sched_chunk": "schedule", "graph": "route", "dag_order": "route",
          "dag_waves": "route", "track_k": "route", "dag_edges": "route", "dag_seq": "route",
-         "sched_kind": "schedule"}
+         "sched_kind": "schedule", "dag_grid": "route", "lshape_groups": "route"}
 
 
 def _measure(design, tuning, rrr, threads):

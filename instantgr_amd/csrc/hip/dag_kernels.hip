@@ -57,6 +57,14 @@ __device__ __forceinline__ double unpack_cost(uint64_t k) {
     return (double)__int_as_float(o >= 0 ? o : o ^ 0x7FFFFFFF);
 }
 
+// Parent-slot keys are updated by 64-bit atomics, which complete in L2 and
+// leave any L1 copy behind; they are read with device-scope loads, which go
+// to L2, so no level has to invalidate the L1 (whose other contents, the DAG
+// and grid inputs, stay cached).
+__device__ __forceinline__ uint64_t load_key(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ double wseg(const GridDev& g, int l, int xmin, int xmax, int ymin, int ymax) {
     return g.presum[gidx(g, l, xmax, ymax)] - g.presum[gidx(g, l, xmin, ymin)];
 }
@@ -156,7 +164,7 @@ __device__ void dp_node_seq(const DagArgs& a, int node) {
         for (int li = low; li < L + 1; li++) {
             if (li > low) via += (double)g.vcost[gidx(g, li - 2, x, y)];
             for (int k = 0; k < nc; k++) {
-                uint64_t key = cur_packed[k * ML + li];
+                uint64_t key = load_key(cur_packed + k * ML + li);
                 double cc = unpack_cost(key);
                 if (cc < mcc[k]) { mcc[k] = cc; bp[k] = (uint64_t)(uint32_t)key * ML + li; }
             }
@@ -409,7 +417,7 @@ __device__ void dp_node16(const DagArgs& a, int node, bool valid, int ncw) {
 #define bp(k) s_bp[k][lane]
 #define cb(k) s_cb[k][lane]
 #pragma unroll
-    for (int k = 0; k < 10; k++) key(k) = (k < nc && act) ? a.packed[(cs + k) * ML + s] : INIT_KEY;
+    for (int k = 0; k < 10; k++) key(k) = (k < nc && act) ? load_key(a.packed + (cs + k) * ML + s) : INIT_KEY;
     // lane t of the group holds the via cut cost of layer t at the node
     const float vl = (valid && s < L) ? g.vcost[gidx(g, s, x, y)] : 0.0f;
     double cst = INF;
@@ -628,12 +636,24 @@ __global__ void __launch_bounds__(TPB) k_dag_edges_pairs(DagArgs a, int64_t edge
 
 // W: waves per SIMD the register allocation must allow (2 leaves the compiler
 // free; 4 caps it at 128 registers, spilling the rest).
+//
+// Persistent form: a grid of at most a few thousand workgroups pulls the
+// batch's nets from a device work queue (one counter per batch, zero at the
+// pass start) in list order, deepest DAG first, so nets of very uneven size
+// (two-pin subnets next to 1000-pin nets' pieces) keep every workgroup busy
+// until the queue is empty; each workgroup exits when the counter passes n.
 template <int W>
 __global__ void __launch_bounds__(WTPB) __attribute__((amdgpu_waves_per_eu(W, 8)))
-k_dag_net(DagArgs a, NetLevels lv, const int* nets, const int* list, const int64_t* node_off,
+k_dag_net(DagArgs a, NetLevels lv, const int* nets, const int* list, int64_t n, int* queue, const int64_t* node_off,
           const int64_t* route_slot, int* routes, int** rptr, int seq_min) {
-    const int64_t i = list[blockIdx.x];
     const int lane = threadIdx.x;
+    for (;;) {
+    __syncthreads();   // the previous net's LDS use is over
+    int qi = 0;
+    if (lane == 0) qi = atomicAdd(queue, 1);
+    qi = __shfl(qi, 0, WTPB);
+    if (qi >= n) break;
+    const int64_t i = list[qi];
     const GridDev& g = a.g;
     const int L = g.L, ML = L + 1;
     const int64_t n0 = node_off[i], n1 = node_off[i + 1];
@@ -667,9 +687,6 @@ k_dag_net(DagArgs a, NetLevels lv, const int* nets, const int* list, const int64
         const int64_t b0 = lv.lvl_edge_off[lb + d], b1 = lv.lvl_edge_off[lb + d + 1];
         for (int64_t k = lane; k < (b1 - b0) * L; k += WTPB) dp_edge_layer(a, lv.dedge[b0 + k / L], (int)(k % L) + 1);
         __syncthreads();
-        // The parent slots were updated by atomics at L2: drop stale L1 lines
-        // before the next level reads them.
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     for (int d = 0; d < nlev; d++) {
         const int64_t a0 = lv.lvl_node_off[lb + d], a1 = lv.lvl_node_off[lb + d + 1];
@@ -695,6 +712,7 @@ k_dag_net(DagArgs a, NetLevels lv, const int* nets, const int* list, const int64
     if (lane == 0) {
         r[0] = base;
         rptr[nets[i]] = r;
+    }
     }
 }
 
@@ -762,14 +780,22 @@ void launch_dag_batch_wave(igr_ctx* c, const DagBatch& b, int stamp) {
     // levels with at least seq_min nodes run a node per lane, narrower ones
     // 16 lanes per node
     const int seq_min = c->tune.dag_seq ? c->tune.dag_seq : DEF_DAG_SEQ;
+    const unsigned grid = (unsigned)std::min<int64_t>(n, c->tune.dag_grid ? c->tune.dag_grid : DEF_DAG_GRID);
+    int* q = D.queue.p + b.index;
     if (c->tune.dag_waves == 4)
-        k_dag_net<4><<<(unsigned)n, WTPB, 0, c->stream>>>(a, lv, D.nets.p, D.glist.p + b.net0, D.node_off.p,
-                                                          D.route_slot.p, c->nets.dag_routes.back()->p,
-                                                          c->nets.rptr.p, seq_min);
+        k_dag_net<4><<<grid, WTPB, 0, c->stream>>>(a, lv, D.nets.p, D.glist.p + b.net0, n, q, D.node_off.p,
+                                                   D.route_slot.p, c->nets.dag_routes.back()->p, c->nets.rptr.p,
+                                                   seq_min);
     else
-        k_dag_net<2><<<(unsigned)n, WTPB, 0, c->stream>>>(a, lv, D.nets.p, D.glist.p + b.net0, D.node_off.p,
-                                                          D.route_slot.p, c->nets.dag_routes.back()->p,
-                                                          c->nets.rptr.p, seq_min);
+        k_dag_net<2><<<grid, WTPB, 0, c->stream>>>(a, lv, D.nets.p, D.glist.p + b.net0, n, q, D.node_off.p,
+                                                   D.route_slot.p, c->nets.dag_routes.back()->p, c->nets.rptr.p,
+                                                   seq_min);
+    HIP_OK(hipGetLastError());
+}
+
+void launch_dag_queue_reset(igr_ctx* c, int64_t batches) {
+    c->dag.queue.alloc(std::max<int64_t>(1, batches));
+    k_fill_i32<<<blocks(std::max<int64_t>(1, batches)), TPB, 0, c->stream>>>(c->dag.queue.p, batches, 0);
     HIP_OK(hipGetLastError());
 }
 

@@ -678,6 +678,7 @@ void dag_pass_batched(igr_ctx* c, const int* nets, int64_t n, int mode, const ig
             B.bytes_dp += l_db[rorder[r]];
         }
         B.levels = maxl;
+        B.index = b;
         B.lvl0 = 0;
         mx_nodes = std::max(mx_nodes, B.node1 - B.node0);
         mx_edges = std::max(mx_edges, B.edge1 - B.edge0);
@@ -742,6 +743,7 @@ void dag_pass_batched(igr_ctx* c, const int* nets, int64_t n, int mode, const ig
     // (rip-up, then commit, per batch).
     {
         BatchedState bs(c);
+        launch_dag_queue_reset(c, (int64_t)batches.size());
         // The whole sequence is recorded as one graph unless kernel timing
         // (events on the stream) is on.
         StreamGraph sg(s, !c->prof.on && (c->tune.graph ? c->tune.graph : DEF_GRAPH) == 1);
@@ -1071,6 +1073,9 @@ int igr_set_tuning(igr_ctx* c, const igr_tuning* t) {
             throw std::invalid_argument("presum_k: 0, 4, 8 or 16");
         if (t->presum_k && t->presum_k * 1024 < std::max(c->g.X, c->g.Y))
             throw std::invalid_argument("presum_k too small for this grid's tracks");
+        if (t->dag_grid < 0) throw std::invalid_argument("dag_grid: >= 0");
+        if (t->lshape_groups != 0 && t->lshape_groups != 1 && t->lshape_groups != 2 && t->lshape_groups != 4)
+            throw std::invalid_argument("lshape_groups: 0, 1, 2 or 4");
         c->tune = *t;
     });
 }
@@ -1087,6 +1092,7 @@ int igr_get_tuning(igr_ctx* c, igr_tuning* t) {
         if (!t->dag_seq) t->dag_seq = DEF_DAG_SEQ;
         if (!t->track_k) t->track_k = std::max(c->g.X, c->g.Y) <= 4096 ? 4 : 8;
         if (!t->presum_k) t->presum_k = 16;
+        if (!t->dag_grid) t->dag_grid = DEF_DAG_GRID;
     });
 }
 

@@ -112,6 +112,7 @@ struct Sched {
     DevBuf<int> base, cnt, c_net, ctrl, lists, rel, cand, cstamp, comm, hits, hstamp;
     DevBuf<int64_t> head, c_off, bsum, cur;
     DevBuf<unsigned long long> stat;
+    DevBuf<int> bits;             // per-net bit sets (inactive, blocked, assigned)
 };
 
 // One rip-up-and-reroute pass on the device: DAGs of the rerouted nets in
@@ -132,8 +133,9 @@ struct DagDev {
     // per-routed-net bases and the level cursors of the device layout.
     DevBuf<int64_t> r_node_off, r_par_off, ebase, sbase, lbase, cur_n, cur_e;
     DevBuf<int> r_nodes, r_par, r_slot, r_child_num, r_depth, rorder, err;
-    // Batched form: the routed nets of each batch in launch order.
-    DevBuf<int> glist;
+    // Batched form: the routed nets of each batch in launch order, and the
+    // per-batch work-queue counters.
+    DevBuf<int> glist, queue;
 };
 
 // Arguments of the device DAG layout (one workgroup per routed net).
@@ -235,7 +237,7 @@ struct igr_ctx {
     igr_dev::Scratch scratch;
     igr_dev::DagDev dag;
     int stamp = 0;
-    igr_tuning tune{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};   // launch-shape overrides (0 = built-in)
+    igr_tuning tune{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};   // launch-shape overrides (0 = built-in)
     double last_dag_prep_s = 0;   // host preparation time of the last DAG pass
     double last_dag_prep_parts[4] = {0, 0, 0, 0};   // fill, batch metadata, upload, allocation
     int64_t cells() const { return (int64_t)g.L * g.X * g.Y; }
@@ -279,6 +281,7 @@ constexpr int DEF_SCHED_CHUNK = 16;
 constexpr int DEF_DAG_ORDER = 1;
 constexpr int DEF_DAG_WAVES = 2;
 constexpr int DEF_DAG_SEQ = 16;
+constexpr int DEF_DAG_GRID = 2048;
 // Batched scheduler: release-driven rounds from this many scheduled nets,
 // active-list rounds below.
 constexpr int64_t DEF_SCHED_RELEASE_NETS = 1500000;
@@ -351,13 +354,17 @@ void launch_rip_wire(igr_ctx* c, const int* d_nets, int64_t n, int stamp);
 struct DagBatch {
     int64_t net0, net1, node0, node1, edge0, edge1, slot0, slot1;
     int levels;
+    int index = 0;              // batch number in the pass (work-queue counter)
     int64_t lvl0;               // first entry of this batch in lvl_off
     double bytes_edges = 0, bytes_dp = 0;   // algorithmic HBM bytes
 };
 void launch_dag_batch_plain(igr_ctx* c, const DagBatch& b, int stamp, const std::vector<int64_t>& h_lvl_off);
 void launch_dag_compact(igr_ctx* c, const DagBatch& b);
-// Batched form: one workgroup per net, all depth levels in one launch.
+// Batched form: persistent workgroups pulling the batch's nets from a work
+// queue, all depth levels of a net in one workgroup; the pass's per-batch
+// queue counters are zeroed first.
 void launch_dag_batch_wave(igr_ctx* c, const DagBatch& b, int stamp);
+void launch_dag_queue_reset(igr_ctx* c, int64_t batches);
 // Batched form: the device layout of a pass's DAGs.
 void launch_dag_layout(igr_ctx* c, const DagLayoutArgs& a, int64_t n);
 void launch_expf(const float* in, float* out, int64_t n, hipStream_t s);

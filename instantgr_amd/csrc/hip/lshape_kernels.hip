@@ -443,13 +443,23 @@ constexpr int lswave_groups(int L, int NC) {
     return lswave_lds(L, NC, 4) <= 60 * 1024 ? 4 : lswave_lds(L, NC, 2) <= 60 * 1024 ? 2 : 1;
 }
 
-template <int L, int NC>
-void lswave_launch(igr_ctx* c, const int* d_nets, int64_t n, int stamp) {
-    constexpr int G = lswave_groups(L, NC);
+template <int L, int NC, int G>
+void lswave_launch_g(igr_ctx* c, const int* d_nets, int64_t n, int stamp) {
     NetsDev& N = c->nets;
     const unsigned grid = (unsigned)((n + G - 1) / G);
     k_lshape_wave<L, NC, G><<<grid, 16 * G, 0, c->stream>>>(c->g, d_nets, n, N.node_off.p, N.node.p, N.parent.p,
                                                             N.route_off.p, N.routes.p, stamp);
+}
+
+// Nets per workgroup: the tuned count, capped by what the LDS holds, or the
+// largest that fits.
+template <int L, int NC>
+void lswave_launch(igr_ctx* c, const int* d_nets, int64_t n, int stamp) {
+    constexpr int GMAX = lswave_groups(L, NC);
+    const int G = c->tune.lshape_groups ? std::min(c->tune.lshape_groups, GMAX) : GMAX;
+    if (G >= 4 && GMAX >= 4) lswave_launch_g<L, NC, (GMAX >= 4 ? 4 : 1)>(c, d_nets, n, stamp);
+    else if (G >= 2 && GMAX >= 2) lswave_launch_g<L, NC, (GMAX >= 2 ? 2 : 1)>(c, d_nets, n, stamp);
+    else lswave_launch_g<L, NC, 1>(c, d_nets, n, stamp);
 }
 
 template <int L>

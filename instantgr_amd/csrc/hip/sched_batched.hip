@@ -86,8 +86,18 @@ struct BArgs {
     int* hstamp;                  // per cell: 1 + the round that listed it in hits
     int64_t* cur;                 // X*Y: no active net before this entry of the cell's list in this batch
     unsigned long long* stat;     // rounds run, nets tested for viability, unassigned at batch starts
+    // per net, one bit each (a few MB even at 10M nets, so the random
+    // lookups of the list scans stay in L2): not active in this batch
+    // (blocked, or assigned), blocked in this batch, assigned
+    uint32_t* inact;
+    uint32_t* blkd;
+    uint32_t* asg;
+    int64_t nwords;
     int X, Y, sparsity;
 };
+
+__device__ __forceinline__ bool bit(const uint32_t* b, int64_t i) { return (b[i >> 5] >> (i & 31)) & 1u; }
+__device__ __forceinline__ void set_bit(uint32_t* b, int64_t i) { atomicOr(&b[i >> 5], 1u << (i & 31)); }
 
 // Active-list rounds: the batch's active list of the round whose state is st.
 __device__ __forceinline__ const int* a_list(const BArgs& a, const int* st, int64_t& na) {
@@ -106,7 +116,8 @@ __device__ __forceinline__ const int* r_list(const BArgs& a, const int* st, int6
     return a.cand;
 }
 
-__device__ __forceinline__ bool active(const BArgs& a, int64_t i) { return a.status[i] == 1 && a.batch[i] == -1; }
+// status 1 and no batch yet: the 'inact' bit clear
+__device__ __forceinline__ bool active(const BArgs& a, int64_t i) { return !bit(a.inact, i); }
 
 // ---- per-cell lists -------------------------------------------------------
 __global__ void k_cell_count(BArgs a, int* cnt) {
@@ -352,6 +363,8 @@ __global__ void __launch_bounds__(TPB) k_a_commit(BArgs a, int64_t g, const int*
             lost = lost || a.map[a.pt[k]] != i;
         if (__ballot(lost) == 0 && lane == 0) {
             a.batch[i] = st[C_CUR];
+            set_bit(a.inact, i);
+            set_bit(a.asg, i);
             atomicAdd(&a.ctrl[2 * C_WORDS], 1);   // nets placed so far
             st[C_COMMIT] = 1;
         }
@@ -379,12 +392,16 @@ __global__ void __launch_bounds__(TPB) k_a_revert(BArgs a, int64_t g) {
             bool h = false;
             if (live && !hit && k < k1) {
                 const int pre = a.map[a.pt[k]];
-                h = pre != -1 && a.batch[pre] != -1;
+                h = pre != -1 && bit(a.asg, pre);
             }
             hit = hit || group_bits(__ballot(h), G) != 0;
             if (__ballot(live && !hit && k + G < k1) == 0) break;
         }
-        if (lg == 0 && live && hit) a.status[i] = 2;
+        if (lg == 0 && live && hit) {
+            a.status[i] = 2;
+            set_bit(a.inact, i);
+            set_bit(a.blkd, i);
+        }
     }
 }
 
@@ -412,7 +429,7 @@ __global__ void __launch_bounds__(TPB) k_a_clear_end(BArgs a, int64_t g, int64_t
             for (int64_t q = t; q < cells; q += stride) {
                 int64_t h = a.head[q];
                 const int64_t e = a.c_off[q + 1];
-                while (h < e && a.batch[a.c_net[h]] != -1) h++;
+                while (h < e && bit(a.asg, a.c_net[h])) h++;
                 a.head[q] = h;
                 const int b = h < e ? a.c_net[h] : -1;
                 a.base[q] = b;
@@ -423,14 +440,19 @@ __global__ void __launch_bounds__(TPB) k_a_clear_end(BArgs a, int64_t g, int64_t
             for (int64_t ps = 0; ps < passes; ps++) {
                 const int64_t u = ps * stride + t;
                 const int i = u < nu ? ul[u] : 0;
-                const bool left = u < nu && a.batch[i] == -1;
+                const bool left = u < nu && !bit(a.asg, i);
                 if (u < nu) a.status[i] = left ? 1 : 0;
                 block_append(left, i, &nx[C_NU], nl, s_app);
+            }
+            // every unassigned net is active again in the next batch
+            for (int64_t w = t; w < a.nwords; w += stride) {
+                a.inact[w] = a.asg[w];
+                a.blkd[w] = 0;
             }
         } else {
             for (int64_t q = t; q < cells; q += stride) {
                 const int m = a.map[q];
-                if (m >= 0 && a.status[m] == 2) a.map[q] = -1;
+                if (m >= 0 && bit(a.blkd, m)) a.map[q] = -1;
             }
             int64_t na;
             const int* al = a_list(a, st, na);
@@ -578,6 +600,8 @@ __global__ void __launch_bounds__(TPB) k_r_commit(BArgs a, int64_t g, const int*
         if (__ballot(lost) == 0 && lane == 0) {
             a.comm[atomicAdd(&st[C_NC], 1)] = i;
             a.batch[i] = st[C_CUR];
+            set_bit(a.inact, i);
+            set_bit(a.asg, i);
             atomicAdd(&a.ctrl[2 * C_WORDS], 1);   // nets placed so far
             st[C_COMMIT] = 1;
         }
@@ -634,7 +658,11 @@ __global__ void __launch_bounds__(TPB) k_r_block(BArgs a, int64_t g) {
         const int64_t e = a.c_off[q + 1];
         for (int64_t k = a.cur[q] + lg; k < e; k += G) {
             const int j = a.c_net[k];
-            if (active(a, j)) a.status[j] = 2;
+            if (active(a, j)) {
+                a.status[j] = 2;
+                set_bit(a.inact, j);
+                set_bit(a.blkd, j);
+            }
         }
     }
 }
@@ -662,7 +690,7 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
             for (int64_t q = t; q < cells; q += stride) {
                 int64_t h = a.head[q];
                 const int64_t e = a.c_off[q + 1];
-                while (h < e && a.batch[a.c_net[h]] != -1) h++;
+                while (h < e && bit(a.asg, a.c_net[h])) h++;
                 a.head[q] = h;
                 a.cur[q] = h;
                 const int b = h < e ? a.c_net[h] : -1;
@@ -674,9 +702,14 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
             for (int64_t ps = 0; ps < passes; ps++) {
                 const int64_t u = ps * stride + t;
                 const int i = u < nu ? ul[u] : 0;
-                const bool left = u < nu && a.batch[i] == -1;
+                const bool left = u < nu && !bit(a.asg, i);
                 if (u < nu) a.status[i] = left ? 1 : 0;
                 block_append(left, i, &nx[C_NU], nl, s_app);
+            }
+            // every unassigned net is active again in the next batch
+            for (int64_t w = t; w < a.nwords; w += stride) {
+                a.inact[w] = a.asg[w];
+                a.blkd[w] = 0;
             }
         } else {
             // UNR independent cells per lane and pass; the pass count is the
@@ -691,7 +724,7 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
                     m[u] = q < cells ? a.map[q] : -1;
                 }
                 int s[UNR];
-                for (int u = 0; u < UNR; u++) s[u] = m[u] >= 0 ? a.status[m[u]] : 0;
+                for (int u = 0; u < UNR; u++) s[u] = m[u] >= 0 && bit(a.blkd, m[u]) ? 2 : 0;
                 for (int u = 0; u < UNR; u++) {
                     const int64_t q = ps * per + u * stride + t;
                     const bool rl = s[u] == 2;
@@ -804,12 +837,15 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
     }
     S.stat.zero(3, st);
     S.ctrl.zero(2 * C_WORDS + 1, st);
+    const int64_t nwords = (n + 31) / 32;
+    S.bits.zero(3 * nwords, st);
     BArgs a{n, S.ids.p, d_pt_off ? S.pids.p : S.ids.p, d_pt_off ? d_pt_off : N.pt_off.p, d_pt_off ? d_pt : N.pt.p,
             N.hs_off.p, N.hs.p, N.vs_off.p, N.vs.p, S.map.p, S.base.p, S.c_off.p, nullptr, S.head.p,
             S.batch.p, S.status.p, S.ctrl.p, S.lists.p,
             release ? S.rel.p : nullptr, release ? S.cand.p : nullptr, release ? S.cstamp.p : nullptr,
             release ? S.comm.p : nullptr, release ? S.hits.p : nullptr, release ? S.hstamp.p : nullptr,
             release ? S.cur.p : nullptr, S.stat.p,
+            (uint32_t*)S.bits.p, (uint32_t*)S.bits.p + nwords, (uint32_t*)S.bits.p + 2 * nwords, nwords,
             c->g.X, c->g.Y,
             (mode == 1 || mode == 2) ? 2 : 15};
     // Per-cell lists of the nets' point cells, highest priority first.

@@ -47,7 +47,8 @@ class RunOpts(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("graph", C.c_int), ("sched_group", C.c_int), ("sched_chunk", C.c_int),
                 ("dag_order", C.c_int), ("dag_waves", C.c_int), ("track_k", C.c_int), ("dag_edges", C.c_int),
-                ("dag_seq", C.c_int), ("sched_kind", C.c_int), ("presum_k", C.c_int)]
+                ("dag_seq", C.c_int), ("sched_kind", C.c_int), ("presum_k", C.c_int), ("dag_grid", C.c_int),
+                ("lshape_groups", C.c_int)]
 
 
 TUNING_FIELDS = [f for f, _ in Tuning._fields_]

@@ -18,8 +18,8 @@ def test_cache_roundtrip(tmp_path):
 
 def test_candidates_cover_fields():
     from instantgr_amd import gpu
-    assert set(autotune.CANDIDATES) == set(gpu.TUNING_FIELDS)
-    assert set(autotune.STAGE) == set(gpu.TUNING_FIELDS)
+    assert set(autotune.CANDIDATES) | autotune.PLAIN_ONLY == set(gpu.TUNING_FIELDS)
+    assert set(autotune.STAGE) | autotune.PLAIN_ONLY == set(gpu.TUNING_FIELDS)
 
 
 @pytest.mark.gpu
@@ -30,11 +30,12 @@ def test_tune_small_design_keeps_routes():
     res = autotune.tune(d, rrr=1, threads=8,
                         candidates={"sched_group": [4, 16], "sched_chunk": [4, 32], "graph": [1, 2],
                                     "dag_order": [1, 2], "dag_waves": [2, 4], "track_k": [4, 8],
-                                    "dag_edges": [1, 2], "dag_seq": [2, 1 << 30], "sched_kind": [1, 2]})
+                                    "dag_edges": [1, 2], "dag_seq": [2, 1 << 30], "sched_kind": [1, 2],
+                                    "dag_grid": [16, 2048], "lshape_groups": [1, 4]})
     t = res["tuning"]
     assert t["sched_group"] in (4, 16) and t["sched_chunk"] in (4, 32)
     assert t["graph"] in (1, 2) and t["dag_order"] in (1, 2) and t["dag_waves"] in (2, 4)
-    assert t["track_k"] in (4, 8)
+    assert t["track_k"] in (4, 8) and t["dag_grid"] in (16, 2048) and t["lshape_groups"] in (1, 4)
     assert all(tr["routes_match"] for tr in res["trials"])
     # the chosen shape, applied, routes as the built-in one does
     r0 = gpu.Router.from_design(d)

@@ -95,3 +95,8 @@ def test_lds_conflict_rate(tmp_path, capsys):
     edges = [l for l in out if l.startswith("k_dag_edges_wave")][0].split()
     assert edges[1] == "2" and edges[-1] == "0.0200"
     assert [l for l in out if l.startswith("k_presum")][0].endswith("no LDS")
+
+
+def test_roofline_without_kernel_timing():
+    r = bench.roofline({}, {})
+    assert r["kernel"] is None and r["frac"] is None and "note" in r
