@@ -59,10 +59,26 @@ int igr_run(igr_ctx* c, igrh_design* h, const igr_run_opts* o, igr_run_stats* st
         double a = now_s();
         if ((rc = igr_congestion_view(c, xs.data(), ys.data()))) return rc;
         if ((rc = igr_ripup_mark(c, 0, is_of.data()))) return rc;
+        // The marked subnets by decreasing HPWL, ties in index order: a
+        // counting sort over the HPWL values (non-negative ints, at most
+        // X + Y), the stable order std::stable_sort would give.
         std::vector<int> nets;
-        for (int64_t i = 0; i < pi.n; i++)
-            if (is_of[i]) nets.push_back((int)i);
-        std::stable_sort(nets.begin(), nets.end(), [&](int x, int y) { return sn.hpwl[x] > sn.hpwl[y]; });
+        {
+            int hmax = 0;
+            int64_t cnt = 0;
+            for (int64_t i = 0; i < pi.n; i++)
+                if (is_of[i]) {
+                    hmax = std::max(hmax, sn.hpwl[i]);
+                    cnt++;
+                }
+            std::vector<int64_t> start((size_t)hmax + 2, 0);
+            for (int64_t i = 0; i < pi.n; i++)
+                if (is_of[i]) start[(size_t)(hmax - sn.hpwl[i]) + 1]++;
+            for (size_t h = 1; h < start.size(); h++) start[h] += start[h - 1];
+            nets.resize((size_t)cnt);
+            for (int64_t i = 0; i < pi.n; i++)
+                if (is_of[i]) nets[(size_t)start[(size_t)(hmax - sn.hpwl[i])]++] = (int)i;
+        }
         S.ripped[it] = (int64_t)nets.size();
         if (nets.empty()) break;
         double b = now_s();

@@ -294,6 +294,82 @@ __global__ void __launch_bounds__(1024) k_track_fused(GridDev g, int s_c, int s_
     if (threadIdx.x == 0) g.dirty[blockIdx.x] = 0;
 }
 
+// Cost update and prefix sums alone (no commit, no rip-up: the full sweep at
+// a pass start and the timed sweep), as a persistent kernel: a grid sized to
+// the workgroups the GPU holds walks the tracks, each workgroup loading its
+// next dirty track's demand and capacity before it computes the current one,
+// so loads stay in flight while the costs, the scan and the stores run.  The
+// exp table is copied to LDS once per workgroup.  The values are those of
+// k_track_fused with s_c = s_r = 0.
+template <int K>
+__global__ void __launch_bounds__(1024) k_track_cost(GridDev g, unsigned long long* stats) {
+    constexpr int NWMAX = 16;
+    const int nw = blockDim.x >> 6;
+    __shared__ float2 sh_f2[NWMAX * 64];
+    __shared__ double lds_d[NWMAX];
+    __shared__ float s_exp[EXPTAB_N];
+    for (int i = threadIdx.x; i < EXPTAB_N; i += blockDim.x) s_exp[i] = g.exptab[i];
+    const int T = g.tbase[g.L];
+    const int p0 = threadIdx.x * K;
+    auto next_dirty = [&](int t) {
+        while (t < T && !g.dirty[t]) t += gridDim.x;
+        return t;
+    };
+    auto geometry = [&](int t, int& l, int& len, int64_t& base) {
+        l = 0;
+        while (l < g.L - 1 && t >= g.tbase[l + 1]) l++;
+        len = layer_dir(l, g.DIR) ? g.Y : g.X;
+        base = (int64_t)l * g.XYc + (int64_t)(t - g.tbase[l]) * len;
+    };
+    int t = next_dirty((int)blockIdx.x);
+    float dem[K], cap[K];
+    int l = 0, len = 0;
+    int64_t base = 0;
+    if (t < T) {
+        geometry(t, l, len, base);
+        load_k<K, float, float4>(g.demand, base, p0, len, (len & 3) == 0, dem);
+        load_k<K, float, float4>(g.cap, base, p0, len, (len & 3) == 0, cap);
+    }
+    while (t < T) {
+        // the next track's loads first
+        const int tn = next_dirty(t + (int)gridDim.x);
+        int ln = 0, lenn = 0;
+        int64_t basen = 0;
+        float demn[K], capn[K];
+        if (tn < T) {
+            geometry(tn, ln, lenn, basen);
+            load_k<K, float, float4>(g.demand, basen, p0, lenn, (lenn & 3) == 0, demn);
+            load_k<K, float, float4>(g.cap, basen, p0, lenn, (lenn & 3) == 0, capn);
+        }
+        const bool v4 = (len & 3) == 0;
+        const float2 pdc = prev_last(make_float2(dem[K - 1], cap[K - 1]), sh_f2);
+        float wc[K], vc[K];
+        double sv[K];
+        bool q = quarter_grid(pdc.y, pdc.x);
+#pragma unroll
+        for (int j = 0; j < K; j++) q = q && quarter_grid(cap[j], dem[j]);
+        if (__all(q)) track_costs<K, true>(g, l, layer_dir(l, g.DIR), len, p0, dem, cap, pdc, s_exp, wc, vc, sv);
+        else track_costs<K, false>(g, l, layer_dir(l, g.DIR), len, p0, dem, cap, pdc, s_exp, wc, vc, sv);
+        store_k<K>(g.wcost, base, p0, len, v4, wc);
+        store_k<K>(g.vcost, base, p0, len, v4, vc);
+        block_sklansky_rt<double, K, NWMAX>(sv, lds_d, nw);
+        store_k<K>(g.presum, base, p0, len, v4, sv);
+        if (threadIdx.x == 0) {
+            g.dirty[t] = 0;
+            atomicAdd(&stats[0], (unsigned long long)len);
+        }
+        t = tn;
+        l = ln;
+        len = lenn;
+        base = basen;
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            dem[j] = demn[j];
+            cap[j] = capn[j];
+        }
+    }
+}
+
 // Rip-up footprint of routed nets: +1/-1 wire differences at the two ends of
 // each wire, +1/-1 covered-cell differences at its first cell and one past
 // its last, one count per via cut, and the dirty flag of every track touched.
@@ -432,6 +508,25 @@ void launch_track_fused(igr_ctx* c, int s_commit, int s_rip, bool cost) {
     const int K = c->tune.track_k ? c->tune.track_k : (ml <= 4096 ? 4 : 8);
     const int B = ((ml + K - 1) / K + 63) / 64 * 64;
     if (B > 1024) throw std::invalid_argument("track longer than the fused kernel's tile");
+    if (!s_commit && !s_rip && cost) {
+        // cost update and prefix sums only: persistent, prefetching kernel
+        int cus = 0;
+        HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+        auto go = [&](auto kern) {
+            int per = 0;
+            HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, B, 0));
+            const int grid = std::max(1, std::min(T, std::max(1, per) * cus));
+            kern<<<grid, B, 0, s>>>(c->g, st);
+        };
+        switch (K) {
+        case 4: go(k_track_cost<4>); break;
+        case 8: go(k_track_cost<8>); break;
+        case 16: go(k_track_cost<16>); break;
+        default: throw std::invalid_argument("track_k must be 4, 8 or 16");
+        }
+        HIP_OK(hipGetLastError());
+        return;
+    }
     switch (K) {
     case 4: k_track_fused<4><<<T, B, 0, s>>>(c->g, s_commit, s_rip, cs, st); break;
     case 8: k_track_fused<8><<<T, B, 0, s>>>(c->g, s_commit, s_rip, cs, st); break;

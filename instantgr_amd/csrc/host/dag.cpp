@@ -583,15 +583,22 @@ void build_dags(const Grid& g, const Subnets& s, const int* nets, int64_t n, con
         cp[c + 1] = cp[c] + (int64_t)chunks[c].points.size();
     }
     out.n = n;
-    out.node_off.resize(n + 1);
-    out.pt_off.resize(n + 1);
-    out.node.resize(cn[nch]);
-    out.child_num.resize(cn[nch]);
-    out.depth.resize(cn[nch]);
-    out.par_off.resize(cn[nch] + 1);
-    out.par.resize(ce[nch]);
-    out.slot.resize(ce[nch]);
-    out.pt.resize(cp[nch]);
+    // Sized with a quarter to spare when they grow, so the next pass (a few
+    // percent larger or smaller) reuses the pages instead of faulting in a
+    // fresh multi-GB allocation.
+    auto size_to = [](auto& v, int64_t m) {
+        if ((int64_t)v.capacity() < m) v.reserve((size_t)(m + m / 4));
+        v.resize((size_t)m);
+    };
+    size_to(out.node_off, n + 1);
+    size_to(out.pt_off, n + 1);
+    size_to(out.node, cn[nch]);
+    size_to(out.child_num, cn[nch]);
+    size_to(out.depth, cn[nch]);
+    size_to(out.par_off, cn[nch] + 1);
+    size_to(out.par, ce[nch]);
+    size_to(out.slot, ce[nch]);
+    size_to(out.pt, cp[nch]);
     next = 0;
     pool.clear();
     for (int t = 0; t < threads; t++)
