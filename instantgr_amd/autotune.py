@@ -138,3 +138,35 @@ def load(path, design_key):
 def apply(router, tuning):
     if tuning:
         router.set_tuning(**tuning)
+
+
+def main(argv=None):
+    """Tune on a sample of a benchmark design and store the result:
+    python -m instantgr_amd.autotune --config 2 --nets 200000 --key config2"""
+    import argparse
+    import sys
+    from .host import Design
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--config", type=int, default=2, help="benchmark design the sample is drawn from")
+    ap.add_argument("--nets", type=int, default=200000, help="nets of the sample")
+    ap.add_argument("--rrr", type=int, default=1)
+    ap.add_argument("--threads", type=int, default=16)
+    ap.add_argument("--reps", type=int, default=1)
+    ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json"))
+    ap.add_argument("--key", default="", help="cache key (default: configN, as bench.py looks it up)")
+    a = ap.parse_args(argv)
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    d = Design.preset(a.config, nets=a.nets).build(a.threads)
+    log("sample of %d nets built in %.1f s" % (a.nets, time.perf_counter() - t0))
+    res = tune(d, rrr=a.rrr, threads=a.threads, reps=a.reps, log=log)
+    res["sample"] = "%d-net sample of configuration %d, %d rip-up pass" % (a.nets, a.config, a.rrr)
+    save(a.out, a.key or "config%d" % a.config, res)
+    print(json.dumps(res["tuning"], sort_keys=True))
+
+
+if __name__ == "__main__":
+    main()

@@ -40,6 +40,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 
 namespace igr_dev {
 
@@ -49,13 +51,15 @@ constexpr int TPB = 256;
 constexpr int ROUNDS = 30;        // rounds per batch (src/database_cuda.hpp:612)
 
 // control state words (two slots of C_WORDS, then the placed-net counter).
-// C_NU / C_UL: length and buffer of the unassigned list.  Active-list rounds:
-// C_NA / C_AL, the same for the batch's active list (C_NA < 0: the unassigned
-// list itself).  Release-driven rounds: C_NR, cells the previous round
-// released (list rel); C_NK, new owners of those cells (list cand); C_NC, nets
-// committed in this round (list comm); C_NH, cells those nets own (list hits).
+// C_NU / C_UL: length and buffer of the nets round 0 tests.  Active-list
+// rounds: the unassigned list, and C_NA / C_AL the same for the batch's
+// active list (C_NA < 0: the unassigned list itself).  Release-driven rounds:
+// the round-0 candidates (below); C_NV0, the nets round 0 found viable (list
+// v0); C_NR, cells the previous round released (list rel); C_NK, new owners
+// of those cells (list cand); C_NC, nets committed in this round (list comm);
+// C_NH, cells those nets own (list hits).
 enum { C_DONE = 0, C_CUR, C_ROUND, C_COMMIT, C_HAVE, C_NVIABLE, C_NU, C_UL, C_NA, C_AL, C_NR, C_NK, C_NC, C_NH,
-       C_WORDS };
+       C_NV0, C_WORDS };
 constexpr int NLISTS = 3;         // net list buffers of n entries each
 
 struct BArgs {
@@ -74,13 +78,14 @@ struct BArgs {
     int* c_net;                   // per-cell lists, highest index first
     int64_t* head;                // X*Y first unassigned entry of each list
     int* batch;
-    int* status;                  // 1 active in this batch, 2 blocked, 0 assigned earlier
     int* ctrl;
     int* lists;                   // NLISTS buffers of n net indices
     // release-driven rounds only
     int* rel;                     // X*Y cells released by the previous round
     int* cand;                    // nets that took a released cell this round
-    int* cstamp;                  // per net: 1 + the round that listed it in cand
+    int* cstamp;                  // per net: 1 + the round that listed it in cand, -(1 + g) when the
+                                  // end of round g listed it as a round-0 candidate
+    int* v0;                      // nets viable in round 0 of the batch
     int* comm;                    // nets committed this round
     int* hits;                    // X*Y cells owned by the nets committed this round
     int* hstamp;                  // per cell: 1 + the round that listed it in hits
@@ -94,7 +99,15 @@ struct BArgs {
     uint32_t* asg;
     int64_t nwords;
     int X, Y, sparsity;
+    // optional per-round record (IGR_SCHED_TRACE): TR_WORDS per round
+    long long* trace;
+    int64_t trace_cap;
 };
+
+// Per-round record: batch, round in the batch, nets tested, viable,
+// committed, cells those own, cells released by the previous round, new
+// owners, device wall clock at the round's last kernel.
+enum { TR_CUR = 0, TR_ROUND, TR_TESTED, TR_VIABLE, TR_COMM, TR_HITS, TR_REL, TR_NEWOWN, TR_CLOCK, TR_WORDS };
 
 __device__ __forceinline__ bool bit(const uint32_t* b, int64_t i) { return (b[i >> 5] >> (i & 31)) & 1u; }
 __device__ __forceinline__ void set_bit(uint32_t* b, int64_t i) { atomicOr(&b[i >> 5], 1u << (i & 31)); }
@@ -106,7 +119,8 @@ __device__ __forceinline__ const int* a_list(const BArgs& a, const int* st, int6
 }
 
 // Release-driven rounds: the nets whose viability the round whose state is st
-// tests: every unassigned net in round 0, the new owners of released cells later.
+// tests: the round-0 candidates in round 0 (every net in the first batch), the
+// new owners of released cells later.
 __device__ __forceinline__ const int* r_list(const BArgs& a, const int* st, int64_t& na) {
     if (st[C_ROUND] == 0) {
         na = st[C_NU];
@@ -269,6 +283,46 @@ __device__ __forceinline__ void block_append(bool want, int v, int* counter, int
     __syncthreads();   // s is reused by the next call
 }
 
+// A workgroup's appends to one list, gathered in LDS over the whole kernel
+// and written with one counter atomic at the end: counter atomics per wave or
+// per pass serialise thousands of waves on the one address each round.  add()
+// may be called by diverged lanes (the ballot covers the active lanes, and the
+// lane that takes the offsets is one of them); entries beyond the LDS
+// capacity go out at once.  Every thread of the workgroup calls init() and
+// flush().
+constexpr int GL_CAP = 2048;
+struct GroupList {
+    int* buf;                     // LDS, GL_CAP entries
+    int* hdr;                     // LDS: count, then the list offset
+    int* counter;
+    int* list;
+    __device__ __forceinline__ void init() {
+        if (threadIdx.x == 0) hdr[0] = 0;
+        __syncthreads();
+    }
+    __device__ __forceinline__ void add(bool want, int v) {
+        const uint64_t m = __ballot(want);
+        if (m == 0) return;
+        const int lane = threadIdx.x & 63;
+        const int leader = __builtin_ctzll(m);
+        int off = 0;
+        if (lane == leader) off = atomicAdd(&hdr[0], __popcll(m));
+        off = __shfl(off, leader);
+        const int pos = off + __popcll(m & ((1ull << lane) - 1));
+        if (want) {
+            if (pos < GL_CAP) buf[pos] = v;
+            else list[atomicAdd(counter, 1)] = v;
+        }
+    }
+    __device__ __forceinline__ void flush() {
+        __syncthreads();
+        const int nb = hdr[0] < GL_CAP ? hdr[0] : GL_CAP;
+        if (threadIdx.x == 0) hdr[1] = nb ? atomicAdd(counter, nb) : 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) list[hdr[1] + i] = buf[i];
+    }
+};
+
 // 3. (both) a wave per viable net claims the neighbour cells (0,-1), (-1,0),
 //    (1,0) of its points and every sparsity-th cell of its segments
 __global__ void __launch_bounds__(TPB) k_b_claim_rest(BArgs a, int64_t g, const int* vlist) {
@@ -398,7 +452,6 @@ __global__ void __launch_bounds__(TPB) k_a_revert(BArgs a, int64_t g) {
             if (__ballot(live && !hit && k + G < k1) == 0) break;
         }
         if (lg == 0 && live && hit) {
-            a.status[i] = 2;
             set_bit(a.inact, i);
             set_bit(a.blkd, i);
         }
@@ -441,7 +494,6 @@ __global__ void __launch_bounds__(TPB) k_a_clear_end(BArgs a, int64_t g, int64_t
                 const int64_t u = ps * stride + t;
                 const int i = u < nu ? ul[u] : 0;
                 const bool left = u < nu && !bit(a.asg, i);
-                if (u < nu) a.status[i] = left ? 1 : 0;
                 block_append(left, i, &nx[C_NU], nl, s_app);
             }
             // every unassigned net is active again in the next batch
@@ -468,6 +520,18 @@ __global__ void __launch_bounds__(TPB) k_a_clear_end(BArgs a, int64_t g, int64_t
         if (!done) {
             int64_t na;
             (void)a_list(a, st, na);
+            if (a.trace && g < a.trace_cap) {
+                long long* rec = a.trace + g * TR_WORDS;
+                rec[TR_CUR] = st[C_CUR];
+                rec[TR_ROUND] = round;
+                rec[TR_TESTED] = na;
+                rec[TR_VIABLE] = st[C_NVIABLE];
+                rec[TR_COMM] = st[C_NC];
+                rec[TR_HITS] = st[C_NH];
+                rec[TR_REL] = st[C_NR];
+                rec[TR_NEWOWN] = st[C_NK];
+                rec[TR_CLOCK] = (long long)wall_clock64();
+            }
             a.stat[0] += 1;
             a.stat[1] += (unsigned long long)na;
             if (round == 0) a.stat[2] += (unsigned long long)st[C_NU];
@@ -515,9 +579,11 @@ __global__ void __launch_bounds__(TPB) k_a_clear_end(BArgs a, int64_t g, int64_t
 //    G entries a step.
 template <int G>
 __global__ void __launch_bounds__(TPB) k_r_claim(BArgs a, int64_t g) {
-    __shared__ int s_app[TPB / 64 + 1];
+    __shared__ int s_buf[GL_CAP], s_hdr[2];
     int* st = a.ctrl + (g & 1) * C_WORDS;
     if (st[C_DONE] || st[C_ROUND] == 0) return;
+    GroupList gl{s_buf, s_hdr, &st[C_NK], a.cand};
+    gl.init();
     const int64_t nr = st[C_NR];
     const int64_t nth = (int64_t)gridDim.x * TPB / G;
     const int lg = threadIdx.x & (G - 1);
@@ -550,8 +616,9 @@ __global__ void __launch_bounds__(TPB) k_r_claim(BArgs a, int64_t g) {
             h += G;
             if (__ballot(!found && h < e) == 0) break;
         }
-        block_append(fresh, owner, &st[C_NK], a.cand, s_app);
+        gl.add(fresh, owner);
     }
+    gl.flush();
 }
 
 // 2. the nets under test (every unassigned net in round 0, the new owners of
@@ -582,7 +649,10 @@ __global__ void __launch_bounds__(TPB) k_r_viable(BArgs a, int64_t g, int* vlist
             fail = fail || group_bits(__ballot(f), G) != 0;
             if (__ballot(!fail && k + G < k1) == 0) break;
         }
-        if (lg == 0 && !fail) vlist[atomicAdd(&st[C_NVIABLE], 1)] = (int)i;
+        if (lg == 0 && !fail) {
+            vlist[atomicAdd(&st[C_NVIABLE], 1)] = (int)i;
+            if (st[C_ROUND] == 0) a.v0[atomicAdd(&st[C_NV0], 1)] = (int)i;
+        }
     }
 }
 
@@ -613,13 +683,16 @@ __global__ void __launch_bounds__(TPB) k_r_commit(BArgs a, int64_t g, const int*
 //     neighbours, sampled segment cells) and lists those it still owns, once
 //     each.
 __global__ void __launch_bounds__(TPB) k_r_hits(BArgs a, int64_t g) {
+    __shared__ int s_buf[GL_CAP], s_hdr[2];
     int* st = a.ctrl + (g & 1) * C_WORDS;
     if (st[C_DONE] || !st[C_COMMIT]) return;
+    GroupList gl{s_buf, s_hdr, &st[C_NH], a.hits};
+    gl.init();
     const int lane = threadIdx.x & 63;
     const int X = a.X, Y = a.Y;
     const int stamp = (int)g + 1;
     auto hit = [&](int q, int c) {
-        if (a.map[q] == c && atomicExch(&a.hstamp[q], stamp) != stamp) a.hits[atomicAdd(&st[C_NH], 1)] = q;
+        gl.add(a.map[q] == c && atomicExch(&a.hstamp[q], stamp) != stamp, q);
     };
     for (int w = (blockIdx.x * TPB + threadIdx.x) >> 6; w < st[C_NC]; w += (gridDim.x * TPB) >> 6) {
         const int c = a.comm[w];
@@ -640,12 +713,14 @@ __global__ void __launch_bounds__(TPB) k_r_hits(BArgs a, int64_t g) {
             for (int y = s0 % Y; y <= e % Y; y += a.sparsity) hit(x * Y + y, c);
         }
     }
+    gl.flush();
 }
 
 // 5b. every active net listing one of those cells sits out the batch (a net
 //     is blocked when one of its point cells is owned by a net committed in
 //     this batch; owners committed in earlier rounds blocked their listers
-//     then, and their cells only shrink).  A group of G lanes per cell.
+//     then, and their cells only shrink).  A group of G lanes per cell (a
+//     wave: the lists of cells in dense regions run to thousands of nets).
 template <int G>
 __global__ void __launch_bounds__(TPB) k_r_block(BArgs a, int64_t g) {
     const int* st = a.ctrl + (g & 1) * C_WORDS;
@@ -659,11 +734,13 @@ __global__ void __launch_bounds__(TPB) k_r_block(BArgs a, int64_t g) {
         for (int64_t k = a.cur[q] + lg; k < e; k += G) {
             const int j = a.c_net[k];
             if (active(a, j)) {
-                a.status[j] = 2;
                 set_bit(a.inact, j);
                 set_bit(a.blkd, j);
             }
         }
+        // every entry from the cursor on is now inactive for the rest of the
+        // batch (a later release or hit of the cell finds no active lister)
+        if (lg == 0) a.cur[q] = e;
     }
 }
 
@@ -674,7 +751,7 @@ __global__ void __launch_bounds__(TPB) k_r_block(BArgs a, int64_t g) {
 //    to a buffer neither current list uses.  Thread 0 writes the successor
 //    control state.
 __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t cells) {
-    __shared__ int s_app[TPB / 64 + 1];
+    __shared__ int s_buf[GL_CAP], s_hdr[2];
     const int* st = a.ctrl + (g & 1) * C_WORDS;
     int* nx = a.ctrl + ((g + 1) & 1) * C_WORDS;
     const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
@@ -683,64 +760,92 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
     const bool commit = st[C_COMMIT] != 0;
     const int round = st[C_ROUND];
     const bool end = !commit || round + 1 >= ROUNDS;
-    const int fresh = 1 - st[C_UL];   // the unassigned-list buffer not in use
+    const int fresh = 1 - st[C_UL];   // the candidate-list buffer not in use
     int* nl = a.lists + (int64_t)fresh * a.n;
     if (!done) {
         if (end) {
-            for (int64_t q = t; q < cells; q += stride) {
-                int64_t h = a.head[q];
-                const int64_t e = a.c_off[q + 1];
-                while (h < e && bit(a.asg, a.c_net[h])) h++;
-                a.head[q] = h;
-                a.cur[q] = h;
-                const int b = h < e ? a.c_net[h] : -1;
-                a.base[q] = b;
-                a.map[q] = b;
-            }
-            const int* ul = a.lists + (int64_t)st[C_UL] * a.n;
-            const int64_t nu = st[C_NU], passes = (nu + stride - 1) / stride;   // the same for every thread
+            // Round 0 of the next batch tests only nets that may own every
+            // point cell of the kept map: a net viable in round 0 of this
+            // batch and still unassigned keeps its cells (a list's head moves
+            // past assigned nets only), and any other net can only become
+            // viable by taking a cell whose head moved.
+            GroupList gl{s_buf, s_hdr, &nx[C_NU], nl};
+            gl.init();
+            const int stamp = -(int)(g + 1);
+            const int64_t passes = (cells + stride - 1) / stride;
             for (int64_t ps = 0; ps < passes; ps++) {
-                const int64_t u = ps * stride + t;
-                const int i = u < nu ? ul[u] : 0;
-                const bool left = u < nu && !bit(a.asg, i);
-                if (u < nu) a.status[i] = left ? 1 : 0;
-                block_append(left, i, &nx[C_NU], nl, s_app);
+                const int64_t q = ps * stride + t;
+                bool want = false;
+                int b = -1;
+                if (q < cells) {
+                    const int64_t h0 = a.head[q], e = a.c_off[q + 1];
+                    int64_t h = h0;
+                    while (h < e && bit(a.asg, a.c_net[h])) h++;
+                    if (h != h0) a.head[q] = h;
+                    a.cur[q] = h;
+                    b = h < e ? a.c_net[h] : -1;
+                    if (h != h0) a.base[q] = b;
+                    a.map[q] = b;
+                    want = h != h0 && b >= 0 && atomicExch(&a.cstamp[b], stamp) != stamp;
+                }
+                gl.add(want, b);
             }
+            const int64_t nv0 = st[C_NV0], vpasses = (nv0 + stride - 1) / stride;
+            for (int64_t ps = 0; ps < vpasses; ps++) {
+                const int64_t u = ps * stride + t;
+                const int i = u < nv0 ? a.v0[u] : 0;
+                const bool want = u < nv0 && !bit(a.asg, i) && atomicExch(&a.cstamp[i], stamp) != stamp;
+                gl.add(want, i);
+            }
+            gl.flush();
             // every unassigned net is active again in the next batch
             for (int64_t w = t; w < a.nwords; w += stride) {
                 a.inact[w] = a.asg[w];
                 a.blkd[w] = 0;
             }
         } else {
-            // UNR independent cells per lane and pass; the pass count is the
-            // same for every thread (block_append needs all of them)
+            // UNR independent cells per lane and pass
+            GroupList gl{s_buf, s_hdr, &nx[C_NR], a.rel};
+            gl.init();
             constexpr int UNR = 4;
             const int64_t per = stride * UNR;
-            const int64_t passes = (cells + per - 1) / per;
-            for (int64_t ps = 0; ps < passes; ps++) {
+            for (int64_t ps = 0; ps * per < cells; ps++) {
                 int m[UNR];
                 for (int u = 0; u < UNR; u++) {
                     const int64_t q = ps * per + u * stride + t;
                     m[u] = q < cells ? a.map[q] : -1;
                 }
-                int s[UNR];
-                for (int u = 0; u < UNR; u++) s[u] = m[u] >= 0 && bit(a.blkd, m[u]) ? 2 : 0;
+                bool rl[UNR];
+                for (int u = 0; u < UNR; u++) rl[u] = m[u] >= 0 && bit(a.blkd, m[u]);
                 for (int u = 0; u < UNR; u++) {
                     const int64_t q = ps * per + u * stride + t;
-                    const bool rl = s[u] == 2;
-                    if (rl) a.map[q] = -1;
-                    block_append(rl, (int)q, &nx[C_NR], a.rel, s_app);
+                    if (rl[u]) a.map[q] = -1;
+                    gl.add(rl[u], (int)q);
                 }
             }
+            gl.flush();
         }
     }
     if (t == 0) {
         if (!done) {
             int64_t na;
             (void)r_list(a, st, na);
+            if (a.trace && g < a.trace_cap) {
+                long long* rec = a.trace + g * TR_WORDS;
+                rec[TR_CUR] = st[C_CUR];
+                rec[TR_ROUND] = round;
+                rec[TR_TESTED] = na;
+                rec[TR_VIABLE] = st[C_NVIABLE];
+                rec[TR_COMM] = st[C_NC];
+                rec[TR_HITS] = st[C_NH];
+                rec[TR_REL] = st[C_NR];
+                rec[TR_NEWOWN] = st[C_NK];
+                rec[TR_CLOCK] = (long long)wall_clock64();
+            }
             a.stat[0] += 1;
             a.stat[1] += (unsigned long long)na;
-            if (round == 0) a.stat[2] += (unsigned long long)st[C_NU];
+            // unassigned nets at the batch start (placed now, less round 0's commits)
+            if (round == 0) a.stat[2] += (unsigned long long)(a.n - (a.ctrl[2 * C_WORDS] - st[C_NC]));
         }
         int d = st[C_DONE], cur = st[C_CUR], r = st[C_ROUND], have = st[C_HAVE] | (commit ? 1 : 0);
         if (!d) {
@@ -761,6 +866,7 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
         nx[C_HAVE] = have;
         nx[C_COMMIT] = 0;
         nx[C_NVIABLE] = 0;
+        nx[C_NV0] = !done && end ? 0 : st[C_NV0];
         nx[C_NK] = 0;
         nx[C_NC] = 0;
         nx[C_NH] = 0;
@@ -821,7 +927,6 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
     S.c_off.alloc(XYc + 1);
     S.cnt.zero(XYc, st);
     S.batch.alloc(n);
-    S.status.alloc(n);
     S.viable.alloc(n);
     S.lists.alloc(NLISTS * n);
     // Release-driven rounds from this many nets (active-list rounds below).
@@ -841,13 +946,22 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
     S.bits.zero(3 * nwords, st);
     BArgs a{n, S.ids.p, d_pt_off ? S.pids.p : S.ids.p, d_pt_off ? d_pt_off : N.pt_off.p, d_pt_off ? d_pt : N.pt.p,
             N.hs_off.p, N.hs.p, N.vs_off.p, N.vs.p, S.map.p, S.base.p, S.c_off.p, nullptr, S.head.p,
-            S.batch.p, S.status.p, S.ctrl.p, S.lists.p,
+            S.batch.p, S.ctrl.p, S.lists.p,
             release ? S.rel.p : nullptr, release ? S.cand.p : nullptr, release ? S.cstamp.p : nullptr,
+            release ? S.lists.p + 2 * n : nullptr,
             release ? S.comm.p : nullptr, release ? S.hits.p : nullptr, release ? S.hstamp.p : nullptr,
             release ? S.cur.p : nullptr, S.stat.p,
             (uint32_t*)S.bits.p, (uint32_t*)S.bits.p + nwords, (uint32_t*)S.bits.p + 2 * nwords, nwords,
             c->g.X, c->g.Y,
-            (mode == 1 || mode == 2) ? 2 : 15};
+            (mode == 1 || mode == 2) ? 2 : 15, nullptr, 0};
+    // IGR_SCHED_TRACE=<file>: append one line per round (the TR_* record)
+    const char* trace_path = std::getenv("IGR_SCHED_TRACE");
+    DevBuf<long long> trace;
+    if (trace_path && *trace_path) {
+        a.trace_cap = 1 << 20;
+        trace.zero(a.trace_cap * TR_WORDS, st);
+        a.trace = trace.p;
+    }
     // Per-cell lists of the nets' point cells, highest priority first.
     k_cell_count<<<blocks(n), TPB, 0, st>>>(a, S.cnt.p);
     const int64_t nb = (XYc + SCAN_BLOCK - 1) / SCAN_BLOCK;
@@ -865,7 +979,6 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
     k_cell_fill<<<blocks(n), TPB, 0, st>>>(a, S.cnt.p);
     k_cell_sort<<<blocks(XYc), TPB, 0, st>>>(a, XYc);
     k_fill_b<<<blocks(n), TPB, 0, st>>>(S.batch.p, n, -1);
-    k_fill_b<<<blocks(n), TPB, 0, st>>>(S.status.p, n, 1);
     k_start<<<blocks(n), TPB, 0, st>>>(a);
     HIP_OK(hipGetLastError());
     // Lanes per net: about half the average point count, at most 16.
@@ -910,7 +1023,7 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
                     k_b_claim_rest<<<wgrid, TPB, 0, st>>>(a, g, S.viable.p);
                     k_r_commit<<<wgrid, TPB, 0, st>>>(a, g, S.viable.p);
                     k_r_hits<<<wgrid, TPB, 0, st>>>(a, g);
-                    k_r_block<16><<<ggrid, TPB, 0, st>>>(a, g);
+                    k_r_block<64><<<ggrid, TPB, 0, st>>>(a, g);
                     k_r_clear_end<<<gm, TPB, 0, st>>>(a, g, XYc);
                 } else {
                     switch (G) {
@@ -948,6 +1061,26 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
     }
     const int nbatch = h_ctrl[(g & 1) * C_WORDS + C_CUR];
     HIP_OK(hipHostFree(h_ctrl));
+    if (a.trace) {
+        const int64_t nrec = std::min<int64_t>(g, a.trace_cap);
+        std::vector<long long> rec((size_t)(nrec * TR_WORDS));
+        HIP_OK(hipMemcpyAsync(rec.data(), a.trace, sizeof(long long) * rec.size(), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        int khz = 0;
+        HIP_OK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+        if (FILE* f = std::fopen(trace_path, "a")) {
+            std::fprintf(f, "# nets %lld rounds %s lanes %d clock_khz %d\n", (long long)n,
+                         release ? "release-driven" : "active-list", G, khz);
+            std::fprintf(f, "g batch round tested viable committed hits released new_owners clock\n");
+            for (int64_t r = 0; r < nrec; r++) {
+                const long long* q = rec.data() + r * TR_WORDS;
+                if (r > 0 && q[TR_CLOCK] == 0) break;   // rounds after the last batch record nothing
+                std::fprintf(f, "%lld %lld %lld %lld %lld %lld %lld %lld %lld %lld\n", (long long)r, q[0], q[1], q[2],
+                             q[3], q[4], q[5], q[6], q[7], q[8]);
+            }
+            std::fclose(f);
+        }
+    }
     {
         unsigned long long sv[3] = {0, 0, 0};
         HIP_OK(hipMemcpyAsync(sv, S.stat.p, sizeof(sv), hipMemcpyDeviceToHost, st));

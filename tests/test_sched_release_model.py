@@ -6,7 +6,10 @@ sched_batched.hip, kernels k_r_*): a released cell goes to the first active
 net of its priority-sorted list, scanned from a per-cell cursor; only those
 new owners are tested for viability; the revert lists the cells the nets
 committed in this round own and blocks the active nets in their lists; the
-end of a round releases the cells of blocked nets.  That this equals the
+end of a round releases the cells of blocked nets; round 0 of a batch tests
+only the candidates the previous batch's end listed (the nets viable in its
+round 0 and still unassigned, and the new first listers of cells whose list
+head moved).  That this equals the
 reference's round-by-round claims (src/database_cuda.hpp:288-434,437-631) is
 an argument about the rules, so it is checked here on the CPU: an executable
 restatement of those kernels' steps, one step per kernel, against the oracle's
@@ -39,6 +42,7 @@ def release_schedule(order, pt_off, pt, hs_off, hs, vs_off, vs, X, Y, mode, skip
     batch = [-1] * n
     status = [1] * n
     unassigned = list(range(n))
+    cand0 = list(range(n))                                   # round-0 candidates
 
     def active(i):
         return status[i] == 1 and batch[i] == -1
@@ -69,7 +73,7 @@ def release_schedule(order, pt_off, pt, hs_off, hs, vs_off, vs, X, Y, mode, skip
         for rnd in range(ROUNDS):
             # k_r_claim: each released cell to its first active lister
             if rnd == 0:
-                test = list(unassigned)
+                test = list(cand0)
             else:
                 test, seen = [], set()
                 for q in rel:
@@ -85,6 +89,10 @@ def release_schedule(order, pt_off, pt, hs_off, hs, vs_off, vs, X, Y, mode, skip
                             break
             # k_r_viable
             viable = [i for i in test if active(i) and all(cmap[p] == i for p in pts[i])]
+            if rnd == 0:
+                # the candidates hold every net viable in round 0
+                assert set(viable) == {i for i in unassigned if all(cmap[p] == i for p in pts[i])}
+                v0 = list(viable)
             # k_b_claim_rest
             for v in viable:
                 for q in rest_cells(v):
@@ -106,18 +114,28 @@ def release_schedule(order, pt_off, pt, hs_off, hs, vs_off, vs, X, Y, mode, skip
                     for j in lists[q][cur[q]:]:
                         if active(j):
                             status[j] = 2
+                    cur[q] = len(lists[q])                   # all inactive from the cursor on
             # k_r_clear_end
             if not committed or rnd + 1 >= ROUNDS:
                 break
             rel = [q for q in range(XY) if cmap[q] >= 0 and status[cmap[q]] == 2]
             for q in rel:
                 cmap[q] = -1
+        cand0, seen = [], set()
         for q in range(XY):
             lst, h = lists[q], head[q]
             while h < len(lst) and batch[lst[h]] != -1:
                 h += 1
+            moved = h != head[q]
             head[q] = cur[q] = h
             cmap[q] = lst[h] if h < len(lst) else -1
+            if moved and cmap[q] >= 0 and cmap[q] not in seen:
+                seen.add(cmap[q])
+                cand0.append(cmap[q])
+        for i in v0:
+            if batch[i] == -1 and i not in seen:
+                seen.add(i)
+                cand0.append(i)
         left = []
         for i in unassigned:
             status[i] = 1 if batch[i] == -1 else 0
