@@ -109,10 +109,10 @@ struct Sched {
     // batched form: per-cell net lists, their heads and the round-0 map; the
     // compacted unassigned / active net lists; release-driven rounds: the
     // released cells, new owners, committed nets and their cells, with stamps
-    DevBuf<int> base, cnt, c_net, ctrl, lists, rel, cand, cstamp, comm, hits, hstamp, blist;
-    DevBuf<int64_t> head, c_off, bsum, cur;
+    DevBuf<int> base, cnt, c_net, ctrl, lists, rel, cand, cstamp, comm, hits, hstamp;
+    DevBuf<int64_t> head, c_off, c_end, bsum, cur;
     DevBuf<unsigned long long> stat;
-    DevBuf<int> bits;             // per-net bit sets (inactive, blocked, assigned, claimed rest cells)
+    DevBuf<int> bits;             // per-net bit sets (inactive, blocked, assigned)
 };
 
 // One rip-up-and-reroute pass on the device: DAGs of the rerouted nets in

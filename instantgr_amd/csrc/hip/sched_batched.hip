@@ -58,9 +58,11 @@ constexpr int ROUNDS = 30;        // rounds per batch (src/database_cuda.hpp:612
 // v0); C_NR, cells the previous round released (list rel); C_NK, new owners
 // of those cells (list cand); C_NC, nets committed in this round (list comm);
 // C_NH, cells those nets own (list hits).
+// C_LASTC, nets placed when the lists were last compacted.
 enum { C_DONE = 0, C_CUR, C_ROUND, C_COMMIT, C_HAVE, C_NVIABLE, C_NU, C_UL, C_NA, C_AL, C_NR, C_NK, C_NC, C_NH,
-       C_NV0, C_WORDS };
+       C_NV0, C_LASTC, C_WORDS };
 constexpr int NLISTS = 3;         // net list buffers of n entries each
+constexpr int COMPACT_DIV = 64;   // release-driven rounds: compact the cell lists every n/64 placed nets
 
 struct BArgs {
     int64_t n;
@@ -75,6 +77,7 @@ struct BArgs {
     int* map;                     // X*Y owner or -1
     int* base;                    // X*Y highest unassigned net listing the cell, or -1
     const int64_t* c_off;         // X*Y+1 per-cell list offsets
+    int64_t* c_end;               // X*Y end of each list (below c_off[q+1] once compacted)
     int* c_net;                   // per-cell lists, highest index first
     int64_t* head;                // X*Y first unassigned entry of each list
     int* batch;
@@ -178,6 +181,7 @@ __global__ void k_cell_sort(BArgs a, int64_t cells) {
         sift_min(v, 0, e);
     }
     a.head[q] = b;
+    a.c_end[q] = b + m;
     if (a.cur) a.cur[q] = b;
     const int top = m > 0 ? v[0] : -1;
     a.base[q] = top;
@@ -323,6 +327,57 @@ struct GroupList {
     }
 };
 
+// Calls f(valid, q) for the sampled cells q of a net's segments (every
+// sparsity-th cell from each segment's start, src/database_cuda.hpp:320-352),
+// the cells of up to 64 segments at a time spread evenly over the wave's
+// lanes (a lane per segment would walk a long segment alone).  Every lane of
+// the wave calls it together; f sees valid = false on the lanes past the last
+// cell and may use ballots.
+template <class F>
+__device__ __forceinline__ void wave_segment_cells(const BArgs& a, int net, F f) {
+    const int lane = threadIdx.x & 63;
+    const int Y = a.Y, sp = a.sparsity;
+    for (int dir = 0; dir < 2; dir++) {
+        const int64_t s0 = dir ? a.vs_off[net] : a.hs_off[net];
+        const int64_t s1 = dir ? a.vs_off[net + 1] : a.hs_off[net + 1];
+        const int* seg = dir ? a.vs : a.hs;
+        for (int64_t sb = s0; sb < s1; sb += 64) {
+            const int64_t k = sb + lane;
+            int cnt = 0, first = 0, step = 0;
+            if (k < s1) {
+                const int b = seg[2 * k], e = seg[2 * k + 1];
+                if (dir == 0) {          // along x at y = b % Y
+                    const int x0 = b / Y, x1 = e / Y;
+                    cnt = x1 >= x0 ? (x1 - x0) / sp + 1 : 0;
+                    first = x0 * Y + b % Y;
+                    step = sp * Y;
+                } else {                 // along y at x = b / Y
+                    const int y0 = b % Y, y1 = e % Y;
+                    cnt = y1 >= y0 ? (y1 - y0) / sp + 1 : 0;
+                    first = (b / Y) * Y + y0;
+                    step = sp;
+                }
+            }
+            int incl = cnt;              // inclusive scan of the cell counts
+            for (int d = 1; d < 64; d <<= 1) {
+                const int v = __shfl_up(incl, d);
+                if (lane >= d) incl += v;
+            }
+            const int total = __shfl(incl, 63);
+            const int excl = incl - cnt;
+            for (int cb = 0; cb < total; cb += 64) {
+                const int c = cb + lane;
+                int sg = 0;              // the first lane whose inclusive count exceeds c
+                for (int b2 = 32; b2 > 0; b2 >>= 1)
+                    if (__shfl(incl, sg + b2 - 1) <= c) sg += b2;
+                const int f0 = __shfl(first, sg), st0 = __shfl(step, sg), ex = __shfl(excl, sg);
+                const bool valid = c < total;
+                f(valid, valid ? f0 + (c - ex) * st0 : 0);
+            }
+        }
+    }
+}
+
 // 3. (both) a wave per viable net claims the neighbour cells (0,-1), (-1,0),
 //    (1,0) of its points and every sparsity-th cell of its segments
 __global__ void __launch_bounds__(TPB) k_b_claim_rest(BArgs a, int64_t g, const int* vlist) {
@@ -340,14 +395,9 @@ __global__ void __launch_bounds__(TPB) k_b_claim_rest(BArgs a, int64_t g, const 
             if (x - 1 >= 0) atomicMax(&a.map[(x - 1) * Y + y], i);
             if (x + 1 < X) atomicMax(&a.map[(x + 1) * Y + y], i);
         }
-        for (int64_t k = a.hs_off[net] + lane; k < a.hs_off[net + 1]; k += 64) {
-            const int s0 = a.hs[2 * k], e = a.hs[2 * k + 1], y = s0 % Y;
-            for (int x = s0 / Y; x <= e / Y; x += a.sparsity) atomicMax(&a.map[x * Y + y], i);
-        }
-        for (int64_t k = a.vs_off[net] + lane; k < a.vs_off[net + 1]; k += 64) {
-            const int s0 = a.vs[2 * k], e = a.vs[2 * k + 1], x = s0 / Y;
-            for (int y = s0 % Y; y <= e % Y; y += a.sparsity) atomicMax(&a.map[x * Y + y], i);
-        }
+        wave_segment_cells(a, net, [&](bool valid, int q) {
+            if (valid) atomicMax(&a.map[q], i);
+        });
     }
 }
 
@@ -481,7 +531,7 @@ __global__ void __launch_bounds__(TPB) k_a_clear_end(BArgs a, int64_t g, int64_t
         if (end) {
             for (int64_t q = t; q < cells; q += stride) {
                 int64_t h = a.head[q];
-                const int64_t e = a.c_off[q + 1];
+                const int64_t e = a.c_end[q];
                 while (h < e && bit(a.asg, a.c_net[h])) h++;
                 a.head[q] = h;
                 const int b = h < e ? a.c_net[h] : -1;
@@ -593,7 +643,7 @@ __global__ void __launch_bounds__(TPB) k_r_claim(BArgs a, int64_t g) {
         const bool live = t < nr;
         const int q = live ? a.rel[t] : 0;
         int64_t h = live ? a.cur[q] : 0;
-        const int64_t e = live ? a.c_off[q + 1] : 0;
+        const int64_t e = live ? a.c_end[q] : 0;
         bool found = !live;
         bool fresh = false;   // this lane took the cell for a net not yet listed this round
         int owner = 0;
@@ -704,14 +754,9 @@ __global__ void __launch_bounds__(TPB) k_r_hits(BArgs a, int64_t g) {
             if (x - 1 >= 0) hit((x - 1) * Y + y, c);
             if (x + 1 < X) hit((x + 1) * Y + y, c);
         }
-        for (int64_t k = a.hs_off[net] + lane; k < a.hs_off[net + 1]; k += 64) {
-            const int s0 = a.hs[2 * k], e = a.hs[2 * k + 1], y = s0 % Y;
-            for (int x = s0 / Y; x <= e / Y; x += a.sparsity) hit(x * Y + y, c);
-        }
-        for (int64_t k = a.vs_off[net] + lane; k < a.vs_off[net + 1]; k += 64) {
-            const int s0 = a.vs[2 * k], e = a.vs[2 * k + 1], x = s0 / Y;
-            for (int y = s0 % Y; y <= e % Y; y += a.sparsity) hit(x * Y + y, c);
-        }
+        wave_segment_cells(a, net, [&](bool valid, int q) {
+            gl.add(valid && a.map[q] == c && atomicExch(&a.hstamp[q], stamp) != stamp, q);
+        });
     }
     gl.flush();
 }
@@ -730,7 +775,7 @@ __global__ void __launch_bounds__(TPB) k_r_block(BArgs a, int64_t g) {
     const int lg = threadIdx.x & (G - 1);
     for (int64_t t = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G; t < nh; t += nth) {
         const int q = a.hits[t];
-        const int64_t e = a.c_off[q + 1];
+        const int64_t e = a.c_end[q];
         for (int64_t k = a.cur[q] + lg; k < e; k += G) {
             const int j = a.c_net[k];
             if (active(a, j)) {
@@ -769,26 +814,75 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
             // batch and still unassigned keeps its cells (a list's head moves
             // past assigned nets only), and any other net can only become
             // viable by taking a cell whose head moved.
+            // Every COMPACT_DIV-th of the nets placed since the last time,
+            // the lists drop their assigned entries (order kept): the scans
+            // of the later rounds then skip no dead entries.
             GroupList gl{s_buf, s_hdr, &nx[C_NU], nl};
             gl.init();
             const int stamp = -(int)(g + 1);
-            const int64_t passes = (cells + stride - 1) / stride;
-            for (int64_t ps = 0; ps < passes; ps++) {
-                const int64_t q = ps * stride + t;
-                bool want = false;
-                int b = -1;
-                if (q < cells) {
-                    const int64_t h0 = a.head[q], e = a.c_off[q + 1];
-                    int64_t h = h0;
-                    while (h < e && bit(a.asg, a.c_net[h])) h++;
-                    if (h != h0) a.head[q] = h;
-                    a.cur[q] = h;
-                    b = h < e ? a.c_net[h] : -1;
-                    if (h != h0) a.base[q] = b;
-                    a.map[q] = b;
-                    want = h != h0 && b >= 0 && atomicExch(&a.cstamp[b], stamp) != stamp;
+            const bool compact = (int64_t)(a.ctrl[2 * C_WORDS] - st[C_LASTC]) * COMPACT_DIV >= a.n;
+            if (compact) {
+                // a group of CG lanes per cell compacts its list in place:
+                // every lane reads its entry before the group's ballot, and
+                // an entry moves to a slot at or below its own
+                constexpr int CG = 16;
+                const int lg = threadIdx.x & (CG - 1);
+                const int64_t ng = stride / CG;
+                const int64_t gpasses = (cells + ng - 1) / ng;
+                for (int64_t ps = 0; ps < gpasses; ps++) {
+                    const int64_t q = ps * ng + t / CG;
+                    const bool live = q < cells;
+                    const int64_t s0 = live ? a.c_off[q] : 0, e = live ? a.c_end[q] : 0;
+                    int64_t w = s0;
+                    for (int64_t k = live ? a.head[q] : 0;; k += CG) {
+                        const int64_t kk = k + lg;
+                        int j = -1;
+                        bool keep = false;
+                        if (kk < e) {
+                            j = a.c_net[kk];
+                            keep = !bit(a.asg, j);
+                        }
+                        const uint64_t m = group_bits(__ballot(keep), CG);
+                        if (keep) a.c_net[w + __popcll(m & ((1ull << lg) - 1))] = j;
+                        w += __popcll(m);
+                        if (__ballot(k + CG < e) == 0) break;
+                    }
+                    int b = -1;
+                    bool want = false;
+                    if (live && lg == 0) {
+                        a.c_end[q] = w;
+                        a.head[q] = s0;
+                        a.cur[q] = s0;
+                        b = w > s0 ? a.c_net[s0] : -1;
+                        const bool moved = b != a.base[q];
+                        if (moved) a.base[q] = b;
+                        a.map[q] = b;
+                        want = moved && b >= 0 && atomicExch(&a.cstamp[b], stamp) != stamp;
+                    }
+                    gl.add(want, b);
                 }
-                gl.add(want, b);
+            } else {
+                const int64_t passes = (cells + stride - 1) / stride;
+                for (int64_t ps = 0; ps < passes; ps++) {
+                    const int64_t q = ps * stride + t;
+                    bool want = false;
+                    int b = -1;
+                    if (q < cells) {
+                        const int64_t h0 = a.head[q], e = a.c_end[q];
+                        int64_t h = h0;
+                        while (h < e && bit(a.asg, a.c_net[h])) h++;
+                        if (h != h0) a.head[q] = h;
+                        b = h < e ? a.c_net[h] : -1;
+                        // the first unassigned lister changed exactly where
+                        // the previous one was assigned
+                        const bool moved = b != a.base[q];
+                        a.cur[q] = h;
+                        if (moved) a.base[q] = b;
+                        a.map[q] = b;
+                        want = moved && b >= 0 && atomicExch(&a.cstamp[b], stamp) != stamp;
+                    }
+                    gl.add(want, b);
+                }
             }
             const int64_t nv0 = st[C_NV0], vpasses = (nv0 + stride - 1) / stride;
             for (int64_t ps = 0; ps < vpasses; ps++) {
@@ -867,6 +961,9 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
         nx[C_COMMIT] = 0;
         nx[C_NVIABLE] = 0;
         nx[C_NV0] = !done && end ? 0 : st[C_NV0];
+        nx[C_LASTC] = !done && end && (int64_t)(a.ctrl[2 * C_WORDS] - st[C_LASTC]) * COMPACT_DIV >= a.n
+                          ? a.ctrl[2 * C_WORDS]
+                          : st[C_LASTC];
         nx[C_NK] = 0;
         nx[C_NC] = 0;
         nx[C_NH] = 0;
@@ -925,6 +1022,7 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
     S.base.alloc(XYc);
     S.head.alloc(XYc);
     S.c_off.alloc(XYc + 1);
+    S.c_end.alloc(XYc);
     S.cnt.zero(XYc, st);
     S.batch.alloc(n);
     S.viable.alloc(n);
@@ -945,7 +1043,7 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
     const int64_t nwords = (n + 31) / 32;
     S.bits.zero(3 * nwords, st);
     BArgs a{n, S.ids.p, d_pt_off ? S.pids.p : S.ids.p, d_pt_off ? d_pt_off : N.pt_off.p, d_pt_off ? d_pt : N.pt.p,
-            N.hs_off.p, N.hs.p, N.vs_off.p, N.vs.p, S.map.p, S.base.p, S.c_off.p, nullptr, S.head.p,
+            N.hs_off.p, N.hs.p, N.vs_off.p, N.vs.p, S.map.p, S.base.p, S.c_off.p, S.c_end.p, nullptr, S.head.p,
             S.batch.p, S.ctrl.p, S.lists.p,
             release ? S.rel.p : nullptr, release ? S.cand.p : nullptr, release ? S.cstamp.p : nullptr,
             release ? S.lists.p + 2 * n : nullptr,

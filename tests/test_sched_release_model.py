@@ -39,6 +39,8 @@ def release_schedule(order, pt_off, pt, hs_off, hs, vs_off, vs, X, Y, mode, skip
     head = [0] * XY
     cur = [0] * XY
     cmap = [lists[q][0] if lists[q] else -1 for q in range(XY)]
+    base = list(cmap)                                        # first unassigned lister
+    last_compact = 0
     batch = [-1] * n
     status = [1] * n
     unassigned = list(range(n))
@@ -122,13 +124,22 @@ def release_schedule(order, pt_off, pt, hs_off, hs, vs_off, vs, X, Y, mode, skip
             for q in rel:
                 cmap[q] = -1
         cand0, seen = [], set()
+        # the lists drop their assigned entries every n/64 placed nets
+        compact = (placed - last_compact) * 64 >= n
+        if compact:
+            last_compact = placed
         for q in range(XY):
-            lst, h = lists[q], head[q]
-            while h < len(lst) and batch[lst[h]] != -1:
-                h += 1
-            moved = h != head[q]
+            if compact:
+                lists[q] = [j for j in lists[q][head[q]:] if batch[j] == -1]
+                h = 0
+            else:
+                lst, h = lists[q], head[q]
+                while h < len(lst) and batch[lst[h]] != -1:
+                    h += 1
             head[q] = cur[q] = h
-            cmap[q] = lst[h] if h < len(lst) else -1
+            cmap[q] = lists[q][h] if h < len(lists[q]) else -1
+            moved = cmap[q] != base[q]
+            base[q] = cmap[q]
             if moved and cmap[q] >= 0 and cmap[q] not in seen:
                 seen.add(cmap[q])
                 cand0.append(cmap[q])
