@@ -106,20 +106,22 @@ def one_step(design, form, rrr, threads, profile=False, tuning=None):
     return r, st
 
 
-def _traffic(kernel):
-    """HBM bytes per dispatch of 'kernel' from the committed counter summary
-    (lines 'traffic <kernel> <bytes>'), or None."""
-    path = os.path.join(PROFILES, "traffic_config1_batched.txt")
+def _traffic(kernel, config):
+    """HBM bytes per dispatch of 'kernel' on benchmark configuration 'config'
+    from the committed counter summary (lines 'traffic <kernel> <bytes>'),
+    and the summary's name; (None, None) when there is none."""
+    name = "traffic_config%d_batched.txt" % config
+    path = os.path.join(PROFILES, name)
     if not kernel or not os.path.exists(path):
-        return None
+        return None, None
     for line in open(path):
         f = line.split()
         if len(f) == 3 and f[0] == "traffic" and f[1] == kernel:
-            return float(f[2])
-    return None
+            return float(f[2]), "profiles/" + name
+    return None, None
 
 
-def roofline(kt, kb):
+def roofline(kt, kb, config):
     """Dominant kernel (by device time in the timed step, among the kernel
     classes with an algorithmic byte model) and its achieved bandwidth on
     those bytes; every class is listed beside it.  The scheduler class is a
@@ -142,7 +144,9 @@ def roofline(kt, kb):
                     "avg_launch_us": ms * 1e3 / launches})
     else:
         out.update({"achieved": None, "frac": None})
-    out["traffic"] = _traffic(KERNEL_SYMBOL.get(name))
+    out["traffic"], src = _traffic(KERNEL_SYMBOL.get(name), config)
+    if src:
+        out["traffic_source"] = src
     classes = {}
     for k, (ms_k, n_k) in kt.items():
         if n_k and ms_k > 0:
@@ -388,12 +392,19 @@ def run_form(design, form, name, a, tuning, budget_s, t_origin):
         r, st = one_step(design, form, a.rrr, a.threads, tuning=tun)
         times.append(st["step_s"])
         est = st["step_s"]
-        _progress("%s step %d: %.3f s" % (name, k, st["step_s"]))
+        free, total = r.device_memory()
+        _progress("%s step %d: %.3f s (device memory in use %.1f of %.1f GB)"
+                  % (name, k, st["step_s"], (total - free) / 1e9, total / 1e9))
         if last is not None:
             last[0].close()
         last = (r, st)
     r, st = last
     routes = r.routes_flat()
+    t_ev = time.perf_counter()
+    gscore = r.evaluate(design)
+    gscore["eval_s"] = time.perf_counter() - t_ev
+    _progress("%s GPU evaluator: %s" % (name, json.dumps(gscore)))
+    r.close()   # before another context takes the device memory of a whole step
     if kt is None and not a.no_profile:
         # no warmup ran: one extra, untimed step with kernel timing
         r2, _ = one_step(design, form, a.rrr, a.threads, profile=True, tuning=tun)
@@ -402,10 +413,6 @@ def run_form(design, form, name, a, tuning, budget_s, t_origin):
         r2.close()
     if kt is not None and not gpu.routes_equal(prof_routes, routes):
         raise RuntimeError("the kernel-timing step routed differently")
-    t_ev = time.perf_counter()
-    gscore = r.evaluate(design)
-    gscore["eval_s"] = time.perf_counter() - t_ev
-    _progress("%s GPU evaluator: %s" % (name, json.dumps(gscore)))
     print("[bench] %s: %s" % (name, json.dumps({"step_s": times, "warmup": warm, "pattern_batches": st["pattern_batches"],
                                                "ripped": st["ripped"], "rrr_batches": st["rrr_batches"],
                                                "tree_s": st["tree_s"], "pattern_s": st["pattern_s"],
@@ -415,7 +422,6 @@ def run_form(design, form, name, a, tuning, budget_s, t_origin):
                                                "rrr_schedule_s": st["rrr_schedule_s"],
                                                "kernel_ms": {k: round(v[0], 1) for k, v in (kt or {}).items() if v[1]}})),
           file=sys.stderr, flush=True)
-    r.close()
     return {"step_s": times, "warmup": warm, "stats": st, "kernels": kt or {}, "bytes": kb or {}, "gpu_score": gscore,
             "routes": routes}
 
@@ -600,7 +606,7 @@ def main():
         out["same_run_comparison"] = comparison
     out["cli"] = cli
     out["cost_prefix_sweep"] = sweeps
-    out["roofline"] = roofline(results[head]["kernels"], results[head]["bytes"])
+    out["roofline"] = roofline(results[head]["kernels"], results[head]["bytes"], a.config)
     if copy_gbs is not None:
         out["roofline"]["copy_probe_GBs"] = copy_gbs
     out["cpu_baseline"] = None if a.no_cpu else cpu_baseline()

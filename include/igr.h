@@ -16,6 +16,7 @@
  */
 #ifndef IGR_H
 #define IGR_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -217,6 +218,10 @@ int igr_kernel_times(igr_ctx* ctx, double* ms, int64_t* launches);
  * enabled, per kernel class (IGR_KCLASSES entries; 0 where no byte model is
  * defined).  The per-unit figures are stated beside each launcher. */
 int igr_kernel_bytes(igr_ctx* ctx, double* bytes);
+
+/* Device memory of the context's GPU: bytes free and total (hipMemGetInfo on
+ * that device).  No counterpart in the reference; for sizing designs. */
+int igr_device_memory(igr_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
 
 /* Launch-shape choices of the batched form; 0 in a field keeps the built-in
  * choice.  No counterpart in the reference (its launch shapes are fixed);

@@ -684,6 +684,23 @@ void dag_pass_batched(igr_ctx* c, const int* nets, int64_t n, int mode, const ig
         mx_edges = std::max(mx_edges, B.edge1 - B.edge0);
         mx_slots = std::max(mx_slots, B.slot1 - B.slot0);
     }
+    // IGR_DAG_TRACE=<file>: append one line per batch (nets, nodes, edges,
+    // slots, deepest DAG's levels, largest DAG's nodes).
+    if (const char* tp = std::getenv("IGR_DAG_TRACE")) {
+        if (FILE* f = std::fopen(tp, "a")) {
+            std::fprintf(f, "# DAG pass: %lld nets, %d batches\nbatch nets nodes edges slots levels max_nodes\n",
+                         (long long)n, nb);
+            for (int b = 0; b < nb; b++) {
+                const DagBatch& B = batches[b];
+                int64_t mxn = 0;
+                for (int64_t r = B.net0; r < B.net1; r++) mxn = std::max(mxn, nbase[r + 1] - nbase[r]);
+                std::fprintf(f, "%d %lld %lld %lld %lld %d %lld\n", b, (long long)(B.net1 - B.net0),
+                             (long long)(B.node1 - B.node0), (long long)(B.edge1 - B.edge0),
+                             (long long)(B.slot1 - B.slot0), B.levels, (long long)mxn);
+            }
+            std::fclose(f);
+        }
+    }
     D.glist.upload(glist.data(), n, s);
     D.nets.upload(rnets.data(), n, s);
     D.rorder.upload(rorder.data(), n, s);
@@ -751,6 +768,7 @@ void dag_pass_batched(igr_ctx* c, const int* nets, int64_t n, int mode, const ig
         launch_rip_scatter(c, D.nets.p + batches[0].net0, batches[0].net1 - batches[0].net0);
         launch_track_fused(c, 0, s1, true);
         for (size_t b = 0; b < batches.size(); b++) {
+            if (b % 512 == 511) progress("DAG pass: %zu of %zu batches enqueued", b + 1, batches.size());
             const int s2 = ++c->stamp;
             launch_dag_batch_wave(c, batches[b], s2);
             if (b + 1 < batches.size()) {
@@ -1102,6 +1120,18 @@ int igr_set_profiling(igr_ctx* c, int on) {
         c->prof.reset();
         c->prof.on = on != 0;
         HIP_OK(hipMemset(c->track_stats.p, 0, 2 * sizeof(unsigned long long)));
+    });
+}
+
+int igr_device_memory(igr_ctx* c, size_t* free_bytes, size_t* total_bytes) {
+    if (!c || !free_bytes || !total_bytes) return fail(IGR_EINVAL, "null argument");
+    return guard([&]() {
+        int prev = 0;
+        HIP_OK(hipGetDevice(&prev));
+        HIP_OK(hipSetDevice(c->device));
+        const hipError_t e = hipMemGetInfo(free_bytes, total_bytes);
+        HIP_OK(hipSetDevice(prev));
+        HIP_OK(e);
     });
 }
 

@@ -6,6 +6,8 @@
 #include "dag.hpp"
 
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <stdexcept>
@@ -83,12 +85,20 @@ void igrh_free(igrh_design* h) { delete h; }
 
 int igrh_build(igrh_design* h, int threads) {
     return guard([&]() {
-        auto t0 = std::chrono::steady_clock::now();
+        using clk = std::chrono::steady_clock;
+        auto sec = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+        const auto t0 = clk::now();
         select_access_points(h->d, h->np, threads);
         build_grid(h->d, h->g);
+        const auto t1 = clk::now();
         build_subnets(h->d, h->np, h->g, h->s, threads);
+        const auto t2 = clk::now();
         prepare_pattern_inputs(h->g, h->s, h->pi, threads);
-        h->t_tree = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        const auto t3 = clk::now();
+        h->t_tree = sec(t0, t3);
+        if (std::getenv("IGR_PROGRESS"))
+            std::fprintf(stderr, "[igr] trees and pattern inputs: access points and grid %.3f s, trees and subnets "
+                                 "%.3f s, pattern inputs %.3f s\n", sec(t0, t1), sec(t1, t2), sec(t2, t3));
         h->built = true;
     });
 }

@@ -118,6 +118,7 @@ SIGNATURES = {
     "igr_congestion_view_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "igr_stamp": (C.c_int, [C.c_void_p]),
     "igr_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "igr_device_memory": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "igr_kernel_times": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "igr_kernel_bytes": (C.c_int, [C.c_void_p, C.c_void_p]),
     "igr_run": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RunOpts), C.POINTER(RunStats)]),
@@ -367,6 +368,12 @@ class Router:
 
     def set_profiling(self, on=True):
         _check(lib().igr_set_profiling(self.h, 1 if on else 0))
+
+    def device_memory(self):
+        """(free, total) bytes of the context's GPU."""
+        free, total = C.c_size_t(), C.c_size_t()
+        _check(lib().igr_device_memory(self.h, C.byref(free), C.byref(total)))
+        return free.value, total.value
 
     def kernel_times(self):
         ms = (C.c_double * len(KCLASSES))()

@@ -17,7 +17,7 @@ def test_roofline_prices_the_dominant_kernel_with_a_byte_model():
     # byte model, so the DAG kernel is priced and the scheduler is named
     kt = {"scheduler": (500.0, 4), "dag_dp": (400.0, 500), "dag_edges": (180.0, 500), "ripup": (60.0, 500)}
     kb = {"dag_dp": 4.0e10, "dag_edges": 2.4e11}
-    r = bench.roofline(kt, kb)
+    r = bench.roofline(kt, kb, 1)
     assert r["kernel"] == "dag_dp" and r["symbol"] == "k_dag_net"
     assert r["largest_class"] == "scheduler"
     assert r["achieved"] == pytest.approx(4.0e10 / 0.4 / 1e9)
@@ -29,20 +29,23 @@ def test_roofline_prices_the_dominant_kernel_with_a_byte_model():
 
 
 def test_roofline_without_largest_class_when_priced_kernel_leads():
-    r = bench.roofline({"dag_dp": (400.0, 500), "scheduler": (100.0, 4)}, {"dag_dp": 4.0e10})
+    r = bench.roofline({"dag_dp": (400.0, 500), "scheduler": (100.0, 4)}, {"dag_dp": 4.0e10}, 1)
     assert r["kernel"] == "dag_dp" and "largest_class" not in r
 
 
-def test_traffic_comes_from_the_committed_counter_summary():
-    path = os.path.join(ROOT, "profiles", "traffic_config1_batched.txt")
+@pytest.mark.parametrize("config", [1, 2])
+def test_traffic_comes_from_the_committed_counter_summary(config):
+    name = "traffic_config%d_batched.txt" % config
+    path = os.path.join(ROOT, "profiles", name)
     want = None
     for line in open(path):
         f = line.split()
         if len(f) == 3 and f[0] == "traffic" and f[1] == "k_dag_net":
             want = float(f[2])
     assert want and want > 0
-    assert bench._traffic("k_dag_net") == want
-    assert bench._traffic("no_such_kernel") is None
+    assert bench._traffic("k_dag_net", config) == (want, "profiles/" + name)
+    assert bench._traffic("no_such_kernel", config) == (None, None)
+    assert bench._traffic("k_dag_net", 7) == (None, None)
 
 
 def test_more_than_one_gpu_is_refused():
@@ -98,5 +101,5 @@ def test_lds_conflict_rate(tmp_path, capsys):
 
 
 def test_roofline_without_kernel_timing():
-    r = bench.roofline({}, {})
+    r = bench.roofline({}, {}, 1)
     assert r["kernel"] is None and r["frac"] is None and "note" in r
