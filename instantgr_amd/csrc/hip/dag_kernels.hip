@@ -144,6 +144,31 @@ __global__ void __launch_bounds__(TPB) k_dag_edges(DagArgs a, int64_t edges) {
 // src/Lshape_route_detour.hpp:281-353): for each lowest layer, the running
 // minimum of every child slot over the layers above it, the via costs in
 // layer order, then the downward sweep.
+// Per-lane DP scratch in LDS for k_dag_net, whose workgroup is one wave
+// (WTPB lanes) running one of the two node DPs at a time: child keys, best
+// child choices, and the current low's choices or running minima.
+// Child-major, so the 64 lanes of one child entry hit consecutive banks; kept
+// out of registers for more waves per SIMD.
+constexpr int WTPB = 64;
+__shared__ uint64_t s_key[10][WTPB], s_bp[10][WTPB], s_cb[10][WTPB];
+
+// The sequential DP's running minima (as bits) and choices of one lane: in
+// registers, or (LDS) in the LDS scratch above, for a one-wave workgroup only
+// (the plain form's k_dag_dp runs 256-thread workgroups).
+template <bool LDS>
+struct SeqScratch {
+    uint64_t m[10], b[10];
+    __device__ __forceinline__ uint64_t& mcc(int k) { return m[k]; }
+    __device__ __forceinline__ uint64_t& bp(int k) { return b[k]; }
+};
+template <>
+struct SeqScratch<true> {
+    __device__ __forceinline__ uint64_t& mcc(int k) { return s_cb[k][threadIdx.x]; }
+    __device__ __forceinline__ uint64_t& bp(int k) { return s_bp[k][threadIdx.x]; }
+};
+
+// Sequential node DP (a lane per node).
+template <bool LDS>
 __device__ void dp_node_seq(const DagArgs& a, int node) {
     const GridDev& g = a.g;
     const int L = g.L, ML = L + 1, XY = g.X * g.Y;
@@ -156,24 +181,30 @@ __device__ void dp_node_seq(const DagArgs& a, int node) {
     uint64_t* cur_best = a.best + cs * ML;
     const uint64_t* cur_packed = a.packed + cs * ML;
     double* cst = a.costs + (int64_t)nr * ML;
-    double mcc[10];
-    uint64_t bp[10];
-    for (int low = 1; low <= fixed_low; low++) {
-        for (int k = 0; k < nc; k++) mcc[k] = INF;
+    SeqScratch<LDS> sc;
+#define mcc(k) sc.mcc(k)
+#define bp(k) sc.bp(k)
+    // low above L visits no layer (the reference's loop runs it empty)
+    const int lowtop = min(fixed_low, L);
+    for (int low = 1; low <= lowtop; low++) {
+        for (int k = 0; k < nc; k++) mcc(k) = __double_as_longlong(INF);
         double via = 0;
         for (int li = low; li < L + 1; li++) {
             if (li > low) via += (double)g.vcost[gidx(g, li - 2, x, y)];
             for (int k = 0; k < nc; k++) {
                 uint64_t key = load_key(cur_packed + k * ML + li);
                 double cc = unpack_cost(key);
-                if (cc < mcc[k]) { mcc[k] = cc; bp[k] = (uint64_t)(uint32_t)key * ML + li; }
+                if (cc < __longlong_as_double(mcc(k))) {
+                    mcc(k) = __double_as_longlong(cc);
+                    bp(k) = (uint64_t)(uint32_t)key * ML + li;
+                }
             }
             if (li >= fixed_high) {
                 double cost = via;
-                for (int k = 0; k < nc; k++) cost += mcc[k];
+                for (int k = 0; k < nc; k++) cost += __longlong_as_double(mcc(k));
                 if (cost < INF && cost < cst[li]) {
                     cst[li] = cost;
-                    for (int k = 0; k < nc; k++) cur_best[k * ML + li] = bp[k];
+                    for (int k = 0; k < nc; k++) cur_best[k * ML + li] = bp(k);
                 }
             }
         }
@@ -183,6 +214,8 @@ __device__ void dp_node_seq(const DagArgs& a, int node) {
                 for (int k = 0; k < nc; k++) cur_best[k * ML + li] = cur_best[k * ML + li + 1];
             }
     }
+#undef mcc
+#undef bp
 }
 
 // Bottom-up DP of one node (thread) of the current depth level.
@@ -195,7 +228,7 @@ __global__ void __launch_bounds__(TPB) k_dag_dp(DagArgs a, const int* lvl_nodes,
     const int nr = (int)(node - a.node0);
     const int x = a.node[node] / g.Y % g.X, y = a.node[node] % g.Y;
     const double* cst = a.costs + (int64_t)nr * ML;
-    dp_node_seq(a, node);
+    dp_node_seq<false>(a, node);
     const uint32_t myid = (uint32_t)a.loc[node] * ID_MUL;
     for (int64_t e = a.par_off[node]; e < a.par_off[node + 1]; e++) {
         const int p = a.par[e], ci = a.slot[e];
@@ -383,8 +416,6 @@ struct NetLevels {
     const int64_t* dedge;      // edges of each level's nodes
 };
 
-constexpr int WTPB = 64;
-
 // Node DP with 16 lanes per node, lane s holding layer index s (1..L).  Same
 // results as dp_node: for each bottom layer 'low' the running minimum of every
 // child slot over layers [low, s] is an inclusive prefix-min across the lanes
@@ -412,7 +443,6 @@ __device__ void dp_node16(const DagArgs& a, int node, bool valid, int ncw) {
     // the best child choices so far (bp) and the current low's (cb), to leave
     // registers for more waves per SIMD
     // (child-major, so the 64 lanes of one child entry hit consecutive banks)
-    __shared__ uint64_t s_key[10][WTPB], s_bp[10][WTPB], s_cb[10][WTPB];
 #define key(k) s_key[k][lane]
 #define bp(k) s_bp[k][lane]
 #define cb(k) s_cb[k][lane]
@@ -424,13 +454,15 @@ __device__ void dp_node16(const DagArgs& a, int node, bool valid, int ncw) {
 #pragma unroll
     for (int k = 0; k < 10; k++) bp(k) = 0;
     // Lowest layer: group-uniform bound, the wave runs to its largest.
-    int lowmax = fixed_low;
+    // (a low above L visits no layer: the reference's loop runs it empty)
+    int lowmax = min(fixed_low, L);
     for (int d = 16; d < 64; d <<= 1) lowmax = max(lowmax, __shfl_xor(lowmax, d, 64));
     for (int low = 1; low <= lowmax; low++) {
         const bool run = act && low <= fixed_low;
         double via = 0;
-#pragma unroll
-        for (int t = 2; t <= MAXL; t++) {
+        // the costs of layers low-1 .. s-2 in ascending order (t runs to L
+        // only: layers from L on are past every lane's s)
+        for (int t = 2; t <= L; t++) {
             const float vt = __shfl(vl, t - 2, 16);
             if (t > low && t <= s) via += (double)vt;
         }
@@ -635,7 +667,8 @@ __global__ void __launch_bounds__(TPB) k_dag_edges_pairs(DagArgs a, int64_t edge
 }
 
 // W: waves per SIMD the register allocation must allow (2 leaves the compiler
-// free; 4 caps it at 128 registers, spilling the rest).
+// free; 4 caps it at 128 registers, spilling the rest).  The LDS scratch
+// (15 KB per one-wave workgroup) allows at most 3, so 4 asks for 3.
 //
 // Persistent form: a grid of at most a few thousand workgroups pulls the
 // batch's nets from a device work queue (one counter per batch, zero at the
@@ -643,7 +676,7 @@ __global__ void __launch_bounds__(TPB) k_dag_edges_pairs(DagArgs a, int64_t edge
 // (two-pin subnets next to 1000-pin nets' pieces) keep every workgroup busy
 // until the queue is empty; each workgroup exits when the counter passes n.
 template <int W>
-__global__ void __launch_bounds__(WTPB) __attribute__((amdgpu_waves_per_eu(W, 8)))
+__global__ void __launch_bounds__(WTPB) __attribute__((amdgpu_waves_per_eu(W < 3 ? W : 3, 8)))
 k_dag_net(DagArgs a, NetLevels lv, const int* nets, const int* list, int64_t n, int* queue, const int64_t* node_off,
           const int64_t* route_slot, int* routes, int** rptr, int seq_min) {
     const int lane = threadIdx.x;
@@ -672,7 +705,7 @@ k_dag_net(DagArgs a, NetLevels lv, const int* nets, const int* list, int64_t n, 
         const int64_t a0 = lv.lvl_node_off[lb + d], a1 = lv.lvl_node_off[lb + d + 1];
         if (a1 - a0 >= seq_min) {
             // wide level: a node per lane
-            for (int64_t k = a0 + lane; k < a1; k += WTPB) dp_node_seq(a, lv.dnode[k]);
+            for (int64_t k = a0 + lane; k < a1; k += WTPB) dp_node_seq<true>(a, lv.dnode[k]);
         } else {
             for (int64_t p = a0; p < a1; p += 4) {
                 const int64_t k = p + (lane >> 4);
