@@ -706,18 +706,28 @@ __global__ void __launch_bounds__(TPB) k_r_viable(BArgs a, int64_t g, int* vlist
     }
 }
 
-// 4. a wave per viable net: commit it if it still owns every point cell
+// 4. a wave per viable net: commit it if it still owns every point cell;
+//    a committed net's wave then lists the cells it owns (points, their
+//    neighbours, sampled segment cells), once each.  Commits do not change
+//    the map, so a net's cells are the same whichever nets of the round
+//    commit around it.
 __global__ void __launch_bounds__(TPB) k_r_commit(BArgs a, int64_t g, const int* vlist) {
+    __shared__ int s_buf[GL_CAP], s_hdr[2];
     int* st = a.ctrl + (g & 1) * C_WORDS;
     if (st[C_DONE]) return;
+    GroupList gl{s_buf, s_hdr, &st[C_NH], a.hits};
+    gl.init();
     const int nv = st[C_NVIABLE];
     const int lane = threadIdx.x & 63;
+    const int X = a.X, Y = a.Y;
+    const int stamp = (int)g + 1;
     for (int w = (blockIdx.x * TPB + threadIdx.x) >> 6; w < nv; w += (gridDim.x * TPB) >> 6) {
         const int i = vlist[w];
+        const int64_t k0 = a.pt_off[a.pids[i]], k1 = a.pt_off[a.pids[i] + 1];
         bool lost = false;
-        for (int64_t k = a.pt_off[a.pids[i]] + lane; k < a.pt_off[a.pids[i] + 1]; k += 64)
-            lost = lost || a.map[a.pt[k]] != i;
-        if (__ballot(lost) == 0 && lane == 0) {
+        for (int64_t k = k0 + lane; k < k1; k += 64) lost = lost || a.map[a.pt[k]] != i;
+        if (__ballot(lost) != 0) continue;
+        if (lane == 0) {
             a.comm[atomicAdd(&st[C_NC], 1)] = i;
             a.batch[i] = st[C_CUR];
             set_bit(a.inact, i);
@@ -725,47 +735,26 @@ __global__ void __launch_bounds__(TPB) k_r_commit(BArgs a, int64_t g, const int*
             atomicAdd(&a.ctrl[2 * C_WORDS], 1);   // nets placed so far
             st[C_COMMIT] = 1;
         }
-    }
-}
-
-// 5a. after a commit: the cells the nets committed in this round own.  A
-//     wave per committed net walks the cells it may own (points, their
-//     neighbours, sampled segment cells) and lists those it still owns, once
-//     each.
-__global__ void __launch_bounds__(TPB) k_r_hits(BArgs a, int64_t g) {
-    __shared__ int s_buf[GL_CAP], s_hdr[2];
-    int* st = a.ctrl + (g & 1) * C_WORDS;
-    if (st[C_DONE] || !st[C_COMMIT]) return;
-    GroupList gl{s_buf, s_hdr, &st[C_NH], a.hits};
-    gl.init();
-    const int lane = threadIdx.x & 63;
-    const int X = a.X, Y = a.Y;
-    const int stamp = (int)g + 1;
-    auto hit = [&](int q, int c) {
-        gl.add(a.map[q] == c && atomicExch(&a.hstamp[q], stamp) != stamp, q);
-    };
-    for (int w = (blockIdx.x * TPB + threadIdx.x) >> 6; w < st[C_NC]; w += (gridDim.x * TPB) >> 6) {
-        const int c = a.comm[w];
-        const int net = a.ids[c];
-        for (int64_t k = a.pt_off[a.pids[c]] + lane; k < a.pt_off[a.pids[c] + 1]; k += 64) {
+        auto hit = [&](int q) { gl.add(a.map[q] == i && atomicExch(&a.hstamp[q], stamp) != stamp, q); };
+        for (int64_t k = k0 + lane; k < k1; k += 64) {
             const int p = a.pt[k], x = p / Y, y = p % Y;
-            hit(p, c);
-            if (y - 1 >= 0) hit(x * Y + y - 1, c);
-            if (x - 1 >= 0) hit((x - 1) * Y + y, c);
-            if (x + 1 < X) hit((x + 1) * Y + y, c);
+            hit(p);
+            if (y - 1 >= 0) hit(x * Y + y - 1);
+            if (x - 1 >= 0) hit((x - 1) * Y + y);
+            if (x + 1 < X) hit((x + 1) * Y + y);
         }
-        wave_segment_cells(a, net, [&](bool valid, int q) {
-            gl.add(valid && a.map[q] == c && atomicExch(&a.hstamp[q], stamp) != stamp, q);
+        wave_segment_cells(a, a.ids[i], [&](bool valid, int q) {
+            gl.add(valid && a.map[q] == i && atomicExch(&a.hstamp[q], stamp) != stamp, q);
         });
     }
     gl.flush();
 }
 
-// 5b. every active net listing one of those cells sits out the batch (a net
-//     is blocked when one of its point cells is owned by a net committed in
-//     this batch; owners committed in earlier rounds blocked their listers
-//     then, and their cells only shrink).  A group of G lanes per cell (a
-//     wave: the lists of cells in dense regions run to thousands of nets).
+// 5. every active net listing one of those cells sits out the batch (a net
+//    is blocked when one of its point cells is owned by a net committed in
+//    this batch; owners committed in earlier rounds blocked their listers
+//    then, and their cells only shrink).  A group of G lanes per cell (a
+//    wave: the lists of cells in dense regions run to thousands of nets).
 template <int G>
 __global__ void __launch_bounds__(TPB) k_r_block(BArgs a, int64_t g) {
     const int* st = a.ctrl + (g & 1) * C_WORDS;
@@ -1120,7 +1109,6 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
                     }
                     k_b_claim_rest<<<wgrid, TPB, 0, st>>>(a, g, S.viable.p);
                     k_r_commit<<<wgrid, TPB, 0, st>>>(a, g, S.viable.p);
-                    k_r_hits<<<wgrid, TPB, 0, st>>>(a, g);
                     k_r_block<64><<<ggrid, TPB, 0, st>>>(a, g);
                     k_r_clear_end<<<gm, TPB, 0, st>>>(a, g, XYc);
                 } else {
