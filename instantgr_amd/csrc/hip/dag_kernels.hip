@@ -184,6 +184,11 @@ __device__ void dp_node_seq(const DagArgs& a, int node) {
     SeqScratch<LDS> sc;
 #define mcc(k) sc.mcc(k)
 #define bp(k) sc.bp(k)
+    // LDS: child 0's keys (read at every low) are loaded once into the lane's
+    // key slots, which the 16-lane DP alone uses otherwise (up to 10 layers)
+    const bool kc = LDS && nc >= 1 && ML <= 10;
+    if (kc)
+        for (int li = 1; li <= L; li++) s_key[li][threadIdx.x] = load_key(cur_packed + li);
     // low above L visits no layer (the reference's loop runs it empty)
     const int lowtop = min(fixed_low, L);
     for (int low = 1; low <= lowtop; low++) {
@@ -192,7 +197,7 @@ __device__ void dp_node_seq(const DagArgs& a, int node) {
         for (int li = low; li < L + 1; li++) {
             if (li > low) via += (double)g.vcost[gidx(g, li - 2, x, y)];
             for (int k = 0; k < nc; k++) {
-                uint64_t key = load_key(cur_packed + k * ML + li);
+                const uint64_t key = k == 0 && kc ? s_key[li][threadIdx.x] : load_key(cur_packed + k * ML + li);
                 double cc = unpack_cost(key);
                 if (cc < __longlong_as_double(mcc(k))) {
                     mcc(k) = __double_as_longlong(cc);
