@@ -112,7 +112,7 @@ struct Sched {
     DevBuf<int> base, cnt, c_net, ctrl, lists, rel, cand, cstamp, comm, hits, hstamp;
     DevBuf<int64_t> head, c_off, c_end, bsum, cur;
     DevBuf<unsigned long long> stat;
-    DevBuf<int> bits;             // per-net bit sets (inactive, blocked, assigned)
+    DevBuf<int> bits;             // per-net state bits (assigned, blocked in this batch)
 };
 
 // One rip-up-and-reroute pass on the device: DAGs of the rerouted nets in

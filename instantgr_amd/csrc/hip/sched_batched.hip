@@ -94,12 +94,10 @@ struct BArgs {
     int* hstamp;                  // per cell: 1 + the round that listed it in hits
     int64_t* cur;                 // X*Y: no active net before this entry of the cell's list in this batch
     unsigned long long* stat;     // rounds run, nets tested for viability, unassigned at batch starts
-    // per net, one bit each (a few MB even at 10M nets, so the random
-    // lookups of the list scans stay in L2): not active in this batch
-    // (blocked, or assigned), blocked in this batch, assigned
-    uint32_t* inact;
-    uint32_t* blkd;
-    uint32_t* asg;
+    // per net two bits, 16 nets a word (a few MB even at 10M nets, so the
+    // random lookups of the list scans stay in L2): bit 0 assigned, bit 1
+    // blocked in this batch; a net is active while both are clear
+    uint32_t* nst;
     int64_t nwords;
     int X, Y, sparsity;
     // optional per-round record (IGR_SCHED_TRACE): TR_WORDS per round
@@ -112,8 +110,19 @@ struct BArgs {
 // owners, device wall clock at the round's last kernel.
 enum { TR_CUR = 0, TR_ROUND, TR_TESTED, TR_VIABLE, TR_COMM, TR_HITS, TR_REL, TR_NEWOWN, TR_CLOCK, TR_WORDS };
 
-__device__ __forceinline__ bool bit(const uint32_t* b, int64_t i) { return (b[i >> 5] >> (i & 31)) & 1u; }
-__device__ __forceinline__ void set_bit(uint32_t* b, int64_t i) { atomicOr(&b[i >> 5], 1u << (i & 31)); }
+__device__ __forceinline__ uint32_t nstate(const BArgs& a, int64_t i) {
+    return (a.nst[i >> 4] >> (2 * (i & 15))) & 3u;
+}
+__device__ __forceinline__ bool assigned(const BArgs& a, int64_t i) { return nstate(a, i) & 1u; }
+__device__ __forceinline__ bool blocked(const BArgs& a, int64_t i) { return nstate(a, i) & 2u; }
+__device__ __forceinline__ void set_assigned(const BArgs& a, int64_t i) {
+    atomicOr(&a.nst[i >> 4], 1u << (2 * (i & 15)));
+}
+__device__ __forceinline__ void set_blocked(const BArgs& a, int64_t i) {
+    atomicOr(&a.nst[i >> 4], 2u << (2 * (i & 15)));
+}
+// batch end: every blocked net is active again (assigned bits kept)
+constexpr uint32_t NST_ASSIGNED = 0x55555555u;
 
 // Active-list rounds: the batch's active list of the round whose state is st.
 __device__ __forceinline__ const int* a_list(const BArgs& a, const int* st, int64_t& na) {
@@ -133,8 +142,8 @@ __device__ __forceinline__ const int* r_list(const BArgs& a, const int* st, int6
     return a.cand;
 }
 
-// status 1 and no batch yet: the 'inact' bit clear
-__device__ __forceinline__ bool active(const BArgs& a, int64_t i) { return !bit(a.inact, i); }
+// not assigned and not blocked in this batch
+__device__ __forceinline__ bool active(const BArgs& a, int64_t i) { return nstate(a, i) == 0; }
 
 // ---- per-cell lists -------------------------------------------------------
 __global__ void k_cell_count(BArgs a, int* cnt) {
@@ -467,8 +476,7 @@ __global__ void __launch_bounds__(TPB) k_a_commit(BArgs a, int64_t g, const int*
             lost = lost || a.map[a.pt[k]] != i;
         if (__ballot(lost) == 0 && lane == 0) {
             a.batch[i] = st[C_CUR];
-            set_bit(a.inact, i);
-            set_bit(a.asg, i);
+            set_assigned(a, i);
             atomicAdd(&a.ctrl[2 * C_WORDS], 1);   // nets placed so far
             st[C_COMMIT] = 1;
         }
@@ -496,14 +504,13 @@ __global__ void __launch_bounds__(TPB) k_a_revert(BArgs a, int64_t g) {
             bool h = false;
             if (live && !hit && k < k1) {
                 const int pre = a.map[a.pt[k]];
-                h = pre != -1 && bit(a.asg, pre);
+                h = pre != -1 && assigned(a, pre);
             }
             hit = hit || group_bits(__ballot(h), G) != 0;
             if (__ballot(live && !hit && k + G < k1) == 0) break;
         }
         if (lg == 0 && live && hit) {
-            set_bit(a.inact, i);
-            set_bit(a.blkd, i);
+            set_blocked(a, i);
         }
     }
 }
@@ -532,7 +539,7 @@ __global__ void __launch_bounds__(TPB) k_a_clear_end(BArgs a, int64_t g, int64_t
             for (int64_t q = t; q < cells; q += stride) {
                 int64_t h = a.head[q];
                 const int64_t e = a.c_end[q];
-                while (h < e && bit(a.asg, a.c_net[h])) h++;
+                while (h < e && assigned(a, a.c_net[h])) h++;
                 a.head[q] = h;
                 const int b = h < e ? a.c_net[h] : -1;
                 a.base[q] = b;
@@ -543,18 +550,15 @@ __global__ void __launch_bounds__(TPB) k_a_clear_end(BArgs a, int64_t g, int64_t
             for (int64_t ps = 0; ps < passes; ps++) {
                 const int64_t u = ps * stride + t;
                 const int i = u < nu ? ul[u] : 0;
-                const bool left = u < nu && !bit(a.asg, i);
+                const bool left = u < nu && !assigned(a, i);
                 block_append(left, i, &nx[C_NU], nl, s_app);
             }
             // every unassigned net is active again in the next batch
-            for (int64_t w = t; w < a.nwords; w += stride) {
-                a.inact[w] = a.asg[w];
-                a.blkd[w] = 0;
-            }
+            for (int64_t w = t; w < a.nwords; w += stride) a.nst[w] &= NST_ASSIGNED;
         } else {
             for (int64_t q = t; q < cells; q += stride) {
                 const int m = a.map[q];
-                if (m >= 0 && bit(a.blkd, m)) a.map[q] = -1;
+                if (m >= 0 && blocked(a, m)) a.map[q] = -1;
             }
             int64_t na;
             const int* al = a_list(a, st, na);
@@ -730,8 +734,7 @@ __global__ void __launch_bounds__(TPB) k_r_commit(BArgs a, int64_t g, const int*
         if (lane == 0) {
             a.comm[atomicAdd(&st[C_NC], 1)] = i;
             a.batch[i] = st[C_CUR];
-            set_bit(a.inact, i);
-            set_bit(a.asg, i);
+            set_assigned(a, i);
             atomicAdd(&a.ctrl[2 * C_WORDS], 1);   // nets placed so far
             st[C_COMMIT] = 1;
         }
@@ -768,8 +771,7 @@ __global__ void __launch_bounds__(TPB) k_r_block(BArgs a, int64_t g) {
         for (int64_t k = a.cur[q] + lg; k < e; k += G) {
             const int j = a.c_net[k];
             if (active(a, j)) {
-                set_bit(a.inact, j);
-                set_bit(a.blkd, j);
+                set_blocked(a, j);
             }
         }
         // every entry from the cursor on is now inactive for the rest of the
@@ -829,7 +831,7 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
                         bool keep = false;
                         if (kk < e) {
                             j = a.c_net[kk];
-                            keep = !bit(a.asg, j);
+                            keep = !assigned(a, j);
                         }
                         const uint64_t m = group_bits(__ballot(keep), CG);
                         if (keep) a.c_net[w + __popcll(m & ((1ull << lg) - 1))] = j;
@@ -859,7 +861,7 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
                     if (q < cells) {
                         const int64_t h0 = a.head[q], e = a.c_end[q];
                         int64_t h = h0;
-                        while (h < e && bit(a.asg, a.c_net[h])) h++;
+                        while (h < e && assigned(a, a.c_net[h])) h++;
                         if (h != h0) a.head[q] = h;
                         b = h < e ? a.c_net[h] : -1;
                         // the first unassigned lister changed exactly where
@@ -877,15 +879,12 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
             for (int64_t ps = 0; ps < vpasses; ps++) {
                 const int64_t u = ps * stride + t;
                 const int i = u < nv0 ? a.v0[u] : 0;
-                const bool want = u < nv0 && !bit(a.asg, i) && atomicExch(&a.cstamp[i], stamp) != stamp;
+                const bool want = u < nv0 && !assigned(a, i) && atomicExch(&a.cstamp[i], stamp) != stamp;
                 gl.add(want, i);
             }
             gl.flush();
             // every unassigned net is active again in the next batch
-            for (int64_t w = t; w < a.nwords; w += stride) {
-                a.inact[w] = a.asg[w];
-                a.blkd[w] = 0;
-            }
+            for (int64_t w = t; w < a.nwords; w += stride) a.nst[w] &= NST_ASSIGNED;
         } else {
             // UNR independent cells per lane and pass
             GroupList gl{s_buf, s_hdr, &nx[C_NR], a.rel};
@@ -899,7 +898,7 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
                     m[u] = q < cells ? a.map[q] : -1;
                 }
                 bool rl[UNR];
-                for (int u = 0; u < UNR; u++) rl[u] = m[u] >= 0 && bit(a.blkd, m[u]);
+                for (int u = 0; u < UNR; u++) rl[u] = m[u] >= 0 && blocked(a, m[u]);
                 for (int u = 0; u < UNR; u++) {
                     const int64_t q = ps * per + u * stride + t;
                     if (rl[u]) a.map[q] = -1;
@@ -1029,8 +1028,8 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
     }
     S.stat.zero(3, st);
     S.ctrl.zero(2 * C_WORDS + 1, st);
-    const int64_t nwords = (n + 31) / 32;
-    S.bits.zero(3 * nwords, st);
+    const int64_t nwords = (n + 15) / 16;
+    S.bits.zero(nwords, st);
     BArgs a{n, S.ids.p, d_pt_off ? S.pids.p : S.ids.p, d_pt_off ? d_pt_off : N.pt_off.p, d_pt_off ? d_pt : N.pt.p,
             N.hs_off.p, N.hs.p, N.vs_off.p, N.vs.p, S.map.p, S.base.p, S.c_off.p, S.c_end.p, nullptr, S.head.p,
             S.batch.p, S.ctrl.p, S.lists.p,
@@ -1038,7 +1037,7 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
             release ? S.lists.p + 2 * n : nullptr,
             release ? S.comm.p : nullptr, release ? S.hits.p : nullptr, release ? S.hstamp.p : nullptr,
             release ? S.cur.p : nullptr, S.stat.p,
-            (uint32_t*)S.bits.p, (uint32_t*)S.bits.p + nwords, (uint32_t*)S.bits.p + 2 * nwords, nwords,
+            (uint32_t*)S.bits.p, nwords,
             c->g.X, c->g.Y,
             (mode == 1 || mode == 2) ? 2 : 15, nullptr, 0};
     // IGR_SCHED_TRACE=<file>: append one line per round (the TR_* record)
