@@ -74,6 +74,9 @@ int igr_create(const igr_grid_desc* d, igr_ctx** out) {
         noalloc_init(c->device);
         HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIP_OK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+        HIP_OK(hipStreamCreateWithFlags(&c->aux_stream, hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
         GridDev& g = c->g;
         g.L = d->L; g.X = d->X; g.Y = d->Y; g.DIR = d->DIR & 1;
         g.XYc = (int64_t)d->X * d->Y;
@@ -115,10 +118,15 @@ void igr_destroy(igr_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
-    hipStream_t s = c->stream, cs = c->copy_stream;
+    if (c->aux_stream) (void)hipStreamSynchronize(c->aux_stream);
+    hipStream_t s = c->stream, cs = c->copy_stream, as = c->aux_stream;
+    hipEvent_t ef = c->ev_fork, ej = c->ev_join;
     delete c;
     if (s) (void)hipStreamDestroy(s);
     if (cs) (void)hipStreamDestroy(cs);
+    if (as) (void)hipStreamDestroy(as);
+    if (ef) (void)hipEventDestroy(ef);
+    if (ej) (void)hipEventDestroy(ej);
 }
 
 static void to_host_ref(igr_ctx* c, const void* dev, void* host, int eb) {
@@ -764,17 +772,32 @@ void dag_pass_batched(igr_ctx* c, const int* nets, int64_t n, int mode, const ig
         // The whole sequence is recorded as one graph unless kernel timing
         // (events on the stream) is on.
         StreamGraph sg(s, !c->prof.on && (c->tune.graph ? c->tune.graph : DEF_GRAPH) == 1);
+        // The rip-up scatter of batch b+1 (the old routes of other nets, into
+        // the rip-up difference arrays) runs on the side stream beside batch
+        // b's DAG kernels (the new routes, into the commit arrays) and joins
+        // before the fused update that consumes both.  Kernel timing keeps
+        // one stream.
+        const bool side = !c->prof.on;
         int s1 = ++c->stamp;
         launch_rip_scatter(c, D.nets.p + batches[0].net0, batches[0].net1 - batches[0].net0);
         launch_track_fused(c, 0, s1, true);
         for (size_t b = 0; b < batches.size(); b++) {
             if (b % 512 == 511) progress("DAG pass: %zu of %zu batches enqueued", b + 1, batches.size());
+            const bool more = b + 1 < batches.size();
+            if (side && more) {
+                const DagBatch& Bn = batches[b + 1];
+                HIP_OK(hipEventRecord(c->ev_fork, s));
+                HIP_OK(hipStreamWaitEvent(c->aux_stream, c->ev_fork, 0));
+                launch_rip_scatter(c, D.nets.p + Bn.net0, Bn.net1 - Bn.net0, c->aux_stream);
+                HIP_OK(hipEventRecord(c->ev_join, c->aux_stream));
+            }
             const int s2 = ++c->stamp;
             launch_dag_batch_wave(c, batches[b], s2);
-            if (b + 1 < batches.size()) {
+            if (more) {
                 const DagBatch& Bn = batches[b + 1];
                 s1 = ++c->stamp;
-                launch_rip_scatter(c, D.nets.p + Bn.net0, Bn.net1 - Bn.net0);
+                if (side) HIP_OK(hipStreamWaitEvent(s, c->ev_join, 0));
+                else launch_rip_scatter(c, D.nets.p + Bn.net0, Bn.net1 - Bn.net0);
                 launch_track_fused(c, s2, s1, true);
             } else {
                 launch_track_fused(c, s2, 0, false);

@@ -222,6 +222,8 @@ struct igr_ctx {
     igr_dev::GridDev g{};
     hipStream_t stream = nullptr;
     hipStream_t copy_stream = nullptr;   // uploads that overlap the compute stream
+    hipStream_t aux_stream = nullptr;    // a DAG pass's rip-up scatters beside the DAG kernels
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int device = 0;
     igr_dev::DevBuf<double> w;
     igr_dev::DevBuf<int> xlen, ylen;
@@ -394,5 +396,5 @@ void gather_routes(igr_ctx* c, const int64_t* d_off, int* d_out);
 // The GPU evaluator (eval_kernels.hip).
 void evaluate_routes_dev(igr_ctx* c, const igr_eval_nets* e, const int64_t* h_route_off, const int* h_routes,
                          igr_score* out);
-void launch_rip_scatter(igr_ctx* c, const int* d_nets, int64_t n);
+void launch_rip_scatter(igr_ctx* c, const int* d_nets, int64_t n, hipStream_t s = nullptr);
 } // namespace igr_dev

@@ -536,10 +536,14 @@ void launch_track_fused(igr_ctx* c, int s_commit, int s_rip, bool cost) {
     HIP_OK(hipGetLastError());
 }
 
-void launch_rip_scatter(igr_ctx* c, const int* d_nets, int64_t n) {
+void launch_rip_scatter(igr_ctx* c, const int* d_nets, int64_t n, hipStream_t s) {
     if (n == 0) return;
-    KScope ks(c, K_RIP);
-    k_rip_scatter<<<blocks(n), SB, 0, c->stream>>>(c->g, d_nets, n, c->nets.rptr.p, c->bad.p);
+    if (s == nullptr || s == c->stream) {
+        KScope ks(c, K_RIP);
+        k_rip_scatter<<<blocks(n), SB, 0, c->stream>>>(c->g, d_nets, n, c->nets.rptr.p, c->bad.p);
+    } else {
+        k_rip_scatter<<<blocks(n), SB, 0, s>>>(c->g, d_nets, n, c->nets.rptr.p, c->bad.p);
+    }
     HIP_OK(hipGetLastError());
 }
 
