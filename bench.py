@@ -124,9 +124,12 @@ def _traffic(kernel, config):
 def roofline(kt, kb, config):
     """Dominant kernel (by device time in the timed step, among the kernel
     classes with an algorithmic byte model) and its achieved bandwidth on
-    those bytes; every class is listed beside it.  The scheduler class is a
-    sequence of small round kernels timed as one group, with no byte model:
-    when it holds the most time it is named in 'largest_class'."""
+    those bytes; every class is listed beside it, with its counter traffic
+    where a summary holds one.  The scheduler class is the sequence of round
+    kernels of one scheduler call, timed as one range; its release-driven
+    rounds count the bytes they move on the device when kernel timing is on.
+    A class without a byte model that holds the most time is named in
+    'largest_class'."""
     if not kt or all(v[0] <= 0 for v in kt.values()):
         return {"kernel": None, "bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "achieved": None, "frac": None,
                 "traffic": None, "note": "kernel timing off (--no-profile)"}
@@ -153,6 +156,10 @@ def roofline(kt, kb, config):
             e = {"ms": round(ms_k, 3), "launches": n_k, "avg_launch_us": round(ms_k * 1e3 / n_k, 2)}
             if kb.get(k, 0) > 0:
                 e["GB/s"] = round(kb[k] / (ms_k * 1e-3) / 1e9, 1)
+                e["bytes_per_launch"] = kb[k] / n_k
+            tr, _ = _traffic(KERNEL_SYMBOL.get(k), config)
+            if tr:
+                e["traffic"] = tr
             classes[k] = e
     out["classes"] = classes
     return out

@@ -103,7 +103,19 @@ struct BArgs {
     // optional per-round record (IGR_SCHED_TRACE): TR_WORDS per round
     long long* trace;
     int64_t trace_cap;
+    // kernel timing only: bytes the release-driven round kernels read and
+    // write, one slot per workgroup index modulo BYTE_SLOTS
+    unsigned long long* bytes;
 };
+constexpr int BYTE_SLOTS = 4096;
+
+// Adds a wave's byte count to its workgroup's slot (profiling only; every
+// lane of the wave calls it together, at the kernel's end).
+__device__ __forceinline__ void flush_bytes(const BArgs& a, unsigned long long nb) {
+    if (!a.bytes) return;
+    for (int off = 32; off > 0; off >>= 1) nb += __shfl_xor(nb, off);
+    if ((threadIdx.x & 63) == 0 && nb) atomicAdd(&a.bytes[blockIdx.x % BYTE_SLOTS], nb);
+}
 
 // Per-round record: batch, round in the batch, nets tested, viable,
 // committed, cells those own, cells released by the previous round, new
@@ -395,19 +407,26 @@ __global__ void __launch_bounds__(TPB) k_b_claim_rest(BArgs a, int64_t g, const 
     const int nv = st[C_NVIABLE];
     const int lane = threadIdx.x & 63;
     const int X = a.X, Y = a.Y;
+    unsigned long long nb = 0;
     for (int w = (blockIdx.x * TPB + threadIdx.x) >> 6; w < nv; w += (gridDim.x * TPB) >> 6) {
         const int i = vlist[w];
         const int net = a.ids[i];
+        if (lane == 0) nb += 4 + 4 + 4 + 16 + 32;   // list entry, id, row, point and segment offsets
         for (int64_t k = a.pt_off[a.pids[i]] + lane; k < a.pt_off[a.pids[i] + 1]; k += 64) {
             const int p = a.pt[k], x = p / Y, y = p % Y;
+            nb += 4 + 12;   // point, three neighbour claims
             if (y - 1 >= 0) atomicMax(&a.map[x * Y + y - 1], i);
             if (x - 1 >= 0) atomicMax(&a.map[(x - 1) * Y + y], i);
             if (x + 1 < X) atomicMax(&a.map[(x + 1) * Y + y], i);
         }
         wave_segment_cells(a, net, [&](bool valid, int q) {
-            if (valid) atomicMax(&a.map[q], i);
+            if (valid) {
+                atomicMax(&a.map[q], i);
+                nb += 4;
+            }
         });
     }
+    flush_bytes(a, nb);
 }
 
 // ---- active-list rounds -----------------------------------------------------
@@ -642,12 +661,14 @@ __global__ void __launch_bounds__(TPB) k_r_claim(BArgs a, int64_t g) {
     const int64_t nth = (int64_t)gridDim.x * TPB / G;
     const int lg = threadIdx.x & (G - 1);
     const int64_t rounds = (nr + nth - 1) / nth;   // the same for every thread
+    unsigned long long nb = 0;
     for (int64_t r = 0; r < rounds; r++) {
         const int64_t t = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G + r * nth;
         const bool live = t < nr;
         const int q = live ? a.rel[t] : 0;
         int64_t h = live ? a.cur[q] : 0;
         const int64_t e = live ? a.c_end[q] : 0;
+        if (live && lg == 0) nb += 4 + 16;   // cell, its cursor and list end
         bool found = !live;
         bool fresh = false;   // this lane took the cell for a net not yet listed this round
         int owner = 0;
@@ -658,6 +679,7 @@ __global__ void __launch_bounds__(TPB) k_r_claim(BArgs a, int64_t g) {
             if (!found && k < e) {
                 j = a.c_net[k];
                 act = active(a, j);
+                nb += 8;   // list entry, state word
             }
             const uint64_t m = group_bits(__ballot(act), G);
             if (m != 0 && lg == __builtin_ctzll(m)) {
@@ -665,6 +687,7 @@ __global__ void __launch_bounds__(TPB) k_r_claim(BArgs a, int64_t g) {
                 a.cur[q] = k;   // the entries before k stay inactive for the rest of the batch
                 owner = j;
                 fresh = atomicExch(&a.cstamp[j], (int)g + 1) != (int)g + 1;
+                nb += 4 + 8 + 8 + (fresh ? 4 : 0);   // map, cursor, stamp, new-owner entry
             }
             found = found || m != 0;
             h += G;
@@ -673,6 +696,7 @@ __global__ void __launch_bounds__(TPB) k_r_claim(BArgs a, int64_t g) {
         gl.add(fresh, owner);
     }
     gl.flush();
+    flush_bytes(a, nb);
 }
 
 // 2. the nets under test (every unassigned net in round 0, the new owners of
@@ -692,22 +716,28 @@ __global__ void __launch_bounds__(TPB) k_r_viable(BArgs a, int64_t g, int* vlist
     const int64_t nth = (int64_t)gridDim.x * TPB / G;
     const int lg = threadIdx.x & (G - 1);
     const int64_t rounds = (na + nth - 1) / nth;   // the same for every lane of the wave
+    unsigned long long nb = 0;
     for (int64_t r = 0; r < rounds; r++) {
         const int64_t t = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G + r * nth;
         const int64_t i = t < na ? al[t] : 0;
         const bool live = t < na && active(a, i);
+        if (t < na && lg == 0) nb += 4 + 4 + (live ? 4 + 16 : 0);   // entry, state, row, point offsets
         const int64_t k0 = live ? a.pt_off[a.pids[i]] : 0, k1 = live ? a.pt_off[a.pids[i] + 1] : 0;
         bool fail = !live;
         for (int64_t k = k0 + lg;; k += G) {
-            const bool f = !fail && k < k1 && a.map[a.pt[k]] != (int)i;
+            const bool chk = !fail && k < k1;
+            const bool f = chk && a.map[a.pt[k]] != (int)i;
+            if (chk) nb += 8;   // point, map cell
             fail = fail || group_bits(__ballot(f), G) != 0;
             if (__ballot(!fail && k + G < k1) == 0) break;
         }
         if (lg == 0 && !fail) {
             vlist[atomicAdd(&st[C_NVIABLE], 1)] = (int)i;
             if (st[C_ROUND] == 0) a.v0[atomicAdd(&st[C_NV0], 1)] = (int)i;
+            nb += st[C_ROUND] == 0 ? 8 : 4;
         }
     }
+    flush_bytes(a, nb);
 }
 
 // 4. a wave per viable net: commit it if it still owns every point cell;
@@ -725,11 +755,16 @@ __global__ void __launch_bounds__(TPB) k_r_commit(BArgs a, int64_t g, const int*
     const int lane = threadIdx.x & 63;
     const int X = a.X, Y = a.Y;
     const int stamp = (int)g + 1;
+    unsigned long long nb = 0;
     for (int w = (blockIdx.x * TPB + threadIdx.x) >> 6; w < nv; w += (gridDim.x * TPB) >> 6) {
         const int i = vlist[w];
         const int64_t k0 = a.pt_off[a.pids[i]], k1 = a.pt_off[a.pids[i] + 1];
+        if (lane == 0) nb += 4 + 4 + 16;   // list entry, row, point offsets
         bool lost = false;
-        for (int64_t k = k0 + lane; k < k1; k += 64) lost = lost || a.map[a.pt[k]] != i;
+        for (int64_t k = k0 + lane; k < k1; k += 64) {
+            lost = lost || a.map[a.pt[k]] != i;
+            nb += 8;   // point, map cell
+        }
         if (__ballot(lost) != 0) continue;
         if (lane == 0) {
             a.comm[atomicAdd(&st[C_NC], 1)] = i;
@@ -737,8 +772,14 @@ __global__ void __launch_bounds__(TPB) k_r_commit(BArgs a, int64_t g, const int*
             set_assigned(a, i);
             atomicAdd(&a.ctrl[2 * C_WORDS], 1);   // nets placed so far
             st[C_COMMIT] = 1;
+            nb += 16 + 16;   // committed entry, batch, state word; segment offsets
         }
-        auto hit = [&](int q) { gl.add(a.map[q] == i && atomicExch(&a.hstamp[q], stamp) != stamp, q); };
+        auto hit = [&](int q) {
+            const bool own = a.map[q] == i;
+            const bool h = own && atomicExch(&a.hstamp[q], stamp) != stamp;
+            nb += 4 + (own ? 8 : 0) + (h ? 4 : 0);   // map cell, stamp, hit entry
+            gl.add(h, q);
+        };
         for (int64_t k = k0 + lane; k < k1; k += 64) {
             const int p = a.pt[k], x = p / Y, y = p % Y;
             hit(p);
@@ -747,10 +788,14 @@ __global__ void __launch_bounds__(TPB) k_r_commit(BArgs a, int64_t g, const int*
             if (x + 1 < X) hit((x + 1) * Y + y);
         }
         wave_segment_cells(a, a.ids[i], [&](bool valid, int q) {
-            gl.add(valid && a.map[q] == i && atomicExch(&a.hstamp[q], stamp) != stamp, q);
+            const bool own = valid && a.map[q] == i;
+            const bool h = own && atomicExch(&a.hstamp[q], stamp) != stamp;
+            if (valid) nb += 4 + (own ? 8 : 0) + (h ? 4 : 0);
+            gl.add(h, q);
         });
     }
     gl.flush();
+    flush_bytes(a, nb);
 }
 
 // 5. every active net listing one of those cells sits out the batch (a net
@@ -765,19 +810,26 @@ __global__ void __launch_bounds__(TPB) k_r_block(BArgs a, int64_t g) {
     const int64_t nh = st[C_NH];
     const int64_t nth = (int64_t)gridDim.x * TPB / G;
     const int lg = threadIdx.x & (G - 1);
+    unsigned long long nb = 0;
     for (int64_t t = ((int64_t)blockIdx.x * TPB + threadIdx.x) / G; t < nh; t += nth) {
         const int q = a.hits[t];
         const int64_t e = a.c_end[q];
         for (int64_t k = a.cur[q] + lg; k < e; k += G) {
             const int j = a.c_net[k];
+            nb += 8;   // list entry, state word
             if (active(a, j)) {
                 set_blocked(a, j);
+                nb += 4;
             }
         }
         // every entry from the cursor on is now inactive for the rest of the
         // batch (a later release or hit of the cell finds no active lister)
-        if (lg == 0) a.cur[q] = e;
+        if (lg == 0) {
+            a.cur[q] = e;
+            nb += 4 + 24;   // hit entry, list end, cursor read and write
+        }
     }
+    flush_bytes(a, nb);
 }
 
 // 6. end of the round.  The batch goes on after a commit below the round
@@ -798,6 +850,7 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
     const bool end = !commit || round + 1 >= ROUNDS;
     const int fresh = 1 - st[C_UL];   // the candidate-list buffer not in use
     int* nl = a.lists + (int64_t)fresh * a.n;
+    unsigned long long nb = 0;
     if (!done) {
         if (end) {
             // Round 0 of the next batch tests only nets that may own every
@@ -829,6 +882,7 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
                         const int64_t kk = k + lg;
                         int j = -1;
                         bool keep = false;
+                        if (kk < e) nb += 12;   // entry read, state word, entry written
                         if (kk < e) {
                             j = a.c_net[kk];
                             keep = !assigned(a, j);
@@ -862,6 +916,7 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
                         const int64_t h0 = a.head[q], e = a.c_end[q];
                         int64_t h = h0;
                         while (h < e && assigned(a, a.c_net[h])) h++;
+                        nb += 16 + 8 * (h - h0 + (h < e ? 1 : 0)) + 8 + 12;   // head, end; entries; cur, base, map
                         if (h != h0) a.head[q] = h;
                         b = h < e ? a.c_net[h] : -1;
                         // the first unassigned lister changed exactly where
@@ -884,7 +939,10 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
             }
             gl.flush();
             // every unassigned net is active again in the next batch
-            for (int64_t w = t; w < a.nwords; w += stride) a.nst[w] &= NST_ASSIGNED;
+            for (int64_t w = t; w < a.nwords; w += stride) {
+                a.nst[w] &= NST_ASSIGNED;
+                nb += 8;
+            }
         } else {
             // UNR independent cells per lane and pass
             GroupList gl{s_buf, s_hdr, &nx[C_NR], a.rel};
@@ -899,6 +957,8 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
                 }
                 bool rl[UNR];
                 for (int u = 0; u < UNR; u++) rl[u] = m[u] >= 0 && blocked(a, m[u]);
+                for (int u = 0; u < UNR; u++)
+                    nb += (ps * per + u * stride + t < cells ? 4 : 0) + (m[u] >= 0 ? 4 : 0) + (rl[u] ? 8 : 0);
                 for (int u = 0; u < UNR; u++) {
                     const int64_t q = ps * per + u * stride + t;
                     if (rl[u]) a.map[q] = -1;
@@ -966,6 +1026,7 @@ __global__ void __launch_bounds__(TPB) k_r_clear_end(BArgs a, int64_t g, int64_t
             nx[C_UL] = st[C_UL];
         }
     }
+    flush_bytes(a, nb);
 }
 
 __global__ void k_fill_b(int* p, int64_t n, int v) {
@@ -1039,7 +1100,13 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
             release ? S.cur.p : nullptr, S.stat.p,
             (uint32_t*)S.bits.p, nwords,
             c->g.X, c->g.Y,
-            (mode == 1 || mode == 2) ? 2 : 15, nullptr, 0};
+            (mode == 1 || mode == 2) ? 2 : 15, nullptr, 0, nullptr};
+    // kernel timing on: the round kernels count the bytes they move
+    DevBuf<unsigned long long> byte_slots;
+    if (c->prof.on && release) {
+        byte_slots.zero(BYTE_SLOTS, st);
+        a.bytes = byte_slots.p;
+    }
     // IGR_SCHED_TRACE=<file>: append one line per round (the TR_* record)
     const char* trace_path = std::getenv("IGR_SCHED_TRACE");
     DevBuf<long long> trace;
@@ -1146,6 +1213,15 @@ int run_schedule_batched(igr_ctx* c, const int* h_order, int64_t n, int mode, in
     }
     const int nbatch = h_ctrl[(g & 1) * C_WORDS + C_CUR];
     HIP_OK(hipHostFree(h_ctrl));
+    if (a.bytes) {
+        std::vector<unsigned long long> slots(BYTE_SLOTS);
+        HIP_OK(hipMemcpyAsync(slots.data(), a.bytes, sizeof(unsigned long long) * BYTE_SLOTS, hipMemcpyDeviceToHost,
+                              st));
+        HIP_OK(hipStreamSynchronize(st));
+        double tot = 0;
+        for (unsigned long long v : slots) tot += (double)v;
+        c->prof.bytes[K_SCHED] += tot;
+    }
     if (a.trace) {
         const int64_t nrec = std::min<int64_t>(g, a.trace_cap);
         std::vector<long long> rec((size_t)(nrec * TR_WORDS));

@@ -28,6 +28,18 @@ def test_roofline_prices_the_dominant_kernel_with_a_byte_model():
     assert "GB/s" in r["classes"]["dag_edges"] and "GB/s" not in r["classes"]["scheduler"]
 
 
+def test_roofline_prices_the_scheduler_when_it_has_bytes():
+    # with the scheduler's device-counted bytes it is the priced leader; the
+    # DAG kernel keeps its counter traffic in the class list
+    kt = {"scheduler": (500.0, 4), "dag_dp": (400.0, 500)}
+    kb = {"scheduler": 2.0e10, "dag_dp": 4.0e10}
+    r = bench.roofline(kt, kb, 2)
+    assert r["kernel"] == "scheduler" and r["symbol"] is None and "largest_class" not in r
+    assert r["achieved"] == pytest.approx(2.0e10 / 0.5 / 1e9)
+    assert r["traffic"] is None
+    assert r["classes"]["dag_dp"]["traffic"] == bench._traffic("k_dag_net", 2)[0]
+
+
 def test_roofline_without_largest_class_when_priced_kernel_leads():
     r = bench.roofline({"dag_dp": (400.0, 500), "scheduler": (100.0, 4)}, {"dag_dp": 4.0e10}, 1)
     assert r["kernel"] == "dag_dp" and "largest_class" not in r
