@@ -216,7 +216,9 @@ int igr_set_profiling(igr_ctx* ctx, int on);
 int igr_kernel_times(igr_ctx* ctx, double* ms, int64_t* launches);
 /* Algorithmic HBM bytes moved by the launches recorded since profiling was
  * enabled, per kernel class (IGR_KCLASSES entries; 0 where no byte model is
- * defined).  The per-unit figures are stated beside each launcher. */
+ * defined).  The per-unit figures are stated beside each launcher; the
+ * scheduler's release-driven rounds count the bytes their kernels read and
+ * write on the device. */
 int igr_kernel_bytes(igr_ctx* ctx, double* bytes);
 
 /* Device memory of the context's GPU: bytes free and total (hipMemGetInfo on
