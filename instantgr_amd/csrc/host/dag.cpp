@@ -54,8 +54,15 @@ struct Builder {
     std::vector<int> region[2];
     std::vector<std::pair<int, int>> ranges[2];
     std::vector<std::vector<int>> stems[2];
-    // DAG under construction
-    std::vector<int> nodes, child_num_pre;
+    // tree nodes' layers and coordinates, decoded once per net
+    struct XY { int x, y; };
+    std::vector<int> tlay;
+    std::vector<XY> txy;
+    // DAG under construction: per node its encoded cell, its coordinates (so
+    // edges never divide the cell index again), its child edges so far and
+    // whether its cell is already in points
+    struct DNode { int cell, x, y, cpre, listed; };
+    std::vector<DNode> dn;
     std::vector<std::pair<int, int>> edges;   // (child, parent*10 + slot) in creation order
     std::vector<int> points;
     // per recursion depth buffers: mirrors created at a node and their places
@@ -126,11 +133,11 @@ struct Builder {
     }
 
     void markCongestion(int x, int par) {
-        int curx = pos[x] / Y % X, cury = pos[x] % Y;
+        const int curx = txy[x].x, cury = txy[x].y;
         for (int64_t a = aoff[x]; a < aoff[x + 1]; a++) {
             int e = tadj[a];
             if (e == par) continue;
-            int ex = pos[e] / Y % X, ey = pos[e] % Y;
+            const int ex = txy[e].x, ey = txy[e].y;
             int dir = -1, congestion = -1;
             if (ex == curx) {
                 if ((ysum[(size_t)curx * Y + std::max(ey, cury)] - ysum[(size_t)curx * Y + std::min(ey, cury)]) > 0) {
@@ -178,7 +185,7 @@ struct Builder {
             for (int64_t a = aoff[x]; a < aoff[x + 1]; a++) {
                 int e = tadj[a];
                 if (e == par) continue;
-                int ex = pos[e] / Y % X, ey = pos[e] % Y;
+                const int ex = txy[e].x, ey = txy[e].y;
                 int er = getRegion(e, dir);
                 if (xr != er) {
                     if (xr >= 0) {
@@ -197,25 +204,34 @@ struct Builder {
     }
 
     int create_node(int l, int x, int y) {
-        int idx = (int)nodes.size();
-        nodes.push_back(l * XYc + x * Y + y);
-        child_num_pre.push_back(0);
+        int idx = (int)dn.size();
+        dn.push_back({l * XYc + x * Y + y, x, y, 0, 0});
         return idx;
     }
 
     void connect_pre(int par_idx, int cur_idx, int child_id) {
         // Edges carry parent*10 + slot (src/database_cuda.hpp:1655).
         if (child_id < 0 || child_id >= 10) throw std::runtime_error("detour DAG node with more than 10 child slots");
-        points.push_back(nodes[cur_idx] % XYc);
-        points.push_back(nodes[par_idx] % XYc);
+        // A node's cell is listed at its first edge only: a repeat would be
+        // dropped by dedup_points anyway, behind the first occurrence.
+        DNode& c = dn[cur_idx];
+        DNode& p = dn[par_idx];
+        if (!c.listed) {
+            c.listed = 1;
+            points.push_back(c.x * Y + c.y);
+        }
+        if (!p.listed) {
+            p.listed = 1;
+            points.push_back(p.x * Y + p.y);
+        }
         edges.emplace_back(cur_idx, par_idx * 10 + child_id);
-        child_num_pre[par_idx]++;
+        p.cpre++;
     }
 
     // src/database_cuda.hpp:1662-1722 (segments are not rebuilt in this pass).
     void connect(int par_idx, int cur_idx, int child_id) {
-        int par_x = nodes[par_idx] / Y % X, par_y = nodes[par_idx] % Y;
-        int cur_x = nodes[cur_idx] / Y % X, cur_y = nodes[cur_idx] % Y;
+        const int par_x = dn[par_idx].x, par_y = dn[par_idx].y;
+        const int cur_x = dn[cur_idx].x, cur_y = dn[cur_idx].y;
         int dist = std::abs(par_x - cur_x) + std::abs(par_y - cur_y);
         if (dist < 50) {
             connect_pre(par_idx, cur_idx, child_id);
@@ -250,7 +266,7 @@ struct Builder {
     void mirror_places(int node, int dir, std::vector<int>& shifts) {
         shifts.clear();
         int reg = region[dir][node];
-        int curx = pos[node] / Y % X, cury = pos[node] % Y;
+        const int curx = txy[node].x, cury = txy[node].y;
         int trunk_len = ranges[dir][reg].second - ranges[dir][reg].first;
         int max_displace = (int)(ratio * (float)trunk_len);
         int orig = dir ? curx : cury;
@@ -282,8 +298,7 @@ struct Builder {
     // place_buf[*][depth-1] (absent for the root).
     void dfs(int x, int par, int par_node, int child_idx, int depth, int level) {
         ensure_depth(level);
-        int cur = pos[x];
-        int curl = cur / XYc, curx = cur / Y % X, cury = cur % Y;
+        const int curl = tlay[x], curx = txy[x].x, cury = txy[x].y;
         int node_idx = create_node(curl, curx, cury);
         std::vector<int>* new_mirrors[2] = {&mir_buf[0][level], &mir_buf[1][level]};
         std::vector<int>* places[2] = {&place_buf[0][level], &place_buf[1][level]};
@@ -309,7 +324,7 @@ struct Builder {
                         nm = create_node(L, nx, ny);
                         connect(mirrors[m], nm, child_idx);
                     } else if (x != root) {
-                        int px = nodes[par_node] / Y % X, py = nodes[par_node] % Y;
+                        const int px = dn[par_node].x, py = dn[par_node].y;
                         if (nx != px && ny != py) {
                             int t[2];
                             for (int pi = 0; pi <= 1; pi++) {
@@ -330,7 +345,7 @@ struct Builder {
             }
         }
         if (par_node != -1) {
-            int px = nodes[par_node] / Y % X, py = nodes[par_node] % Y;
+            const int px = dn[par_node].x, py = dn[par_node].y;
             if (px != curx && py != cury) {
                 for (int pi = 0; pi <= 1; pi++) {
                     int pre = create_node(L, pi ? curx : px, pi ? py : cury);
@@ -384,7 +399,7 @@ struct Builder {
                 if (prid >= 0 && rid != prid) {
                     const std::vector<int>& mirrors = mir_buf[dir][level - 1];
                     for (int pm : mirrors) {
-                        int qx = nodes[pm] / Y % X, qy = nodes[pm] % Y;
+                        const int qx = dn[pm].x, qy = dn[pm].y;
                         if (qx != curx && qy != cury) {
                             for (int pi = 0; pi <= 1; pi++) {
                                 int ins = create_node(L, pi ? curx : qx, pi ? qy : cury);
@@ -417,14 +432,33 @@ struct Builder {
         }
     }
 
+    // The lines of v's next 'bytes' bytes within its capacity, fetched for
+    // writing.
+    static void prefetch_tail(const RawVec<int>& v, size_t bytes) {
+        const char* p = (const char*)(v.data() + v.size());
+        const char* e = (const char*)(v.data() + v.capacity());
+        for (size_t o = 0; o < bytes && p + o < e; o += 64) __builtin_prefetch(p + o, 1);
+    }
+
     // Build one net's DAG and append it to the chunk output.
     void build(const int* tnode, int tn_, const int64_t* adj_off_abs, const int* tadj_, DagChunk& out) {
+        // The chunk arrays were last written a whole build ago: their next
+        // lines (a typical net's share) load while the DAG is being built.
+        for (auto* v : {&out.node, &out.child_num, &out.depth, &out.npar}) prefetch_tail(*v, 384);
+        for (auto* v : {&out.par, &out.slot}) prefetch_tail(*v, 512);
+        prefetch_tail(out.points, 256);
         tn = tn_;
         pos = tnode;
         tadj = tadj_;
         rel.assign(adj_off_abs, adj_off_abs + tn + 1);
         for (auto& v : rel) v -= adj_off_abs[0];
         aoff = rel.data();
+        tlay.resize(tn);
+        txy.resize(tn);
+        for (int x = 0; x < tn; x++) {
+            tlay[x] = pos[x] / XYc;
+            txy[x] = {pos[x] / Y % X, pos[x] % Y};
+        }
         root = select_root();
         for (int d = 0; d < 2; d++) {
             region[d].assign(tn, -1);
@@ -433,7 +467,7 @@ struct Builder {
             for (int x = 0; x < tn; x++) stems[d][x].clear();
         }
         for (int x = 0; x < tn; x++) {
-            int curl = pos[x] / XYc, curx = pos[x] / Y % X, cury = pos[x] % Y;
+            const int curl = tlay[x], curx = txy[x].x, cury = txy[x].y;
             for (int dir = 0; dir < 2; dir++) {
                 if (curl < L) stems[dir][x].push_back(dir ? curx : cury);
                 int t = dir ? cury : curx;
@@ -441,15 +475,14 @@ struct Builder {
             }
         }
         markCongestion(root, -1);
-        nodes.clear();
-        child_num_pre.clear();
+        dn.clear();
         edges.clear();
         points.clear();
         dfs(root, -1, -1, -1, 0, 0);
         dedup_points();
 
         // Parent lists in creation order (CSR by child).
-        const int n = (int)nodes.size();
+        const int n = (int)dn.size();
         poff.assign(n + 1, 0);
         for (auto& e : edges) poff[e.first + 1]++;
         for (int i = 0; i < n; i++) poff[i + 1] += poff[i];
@@ -460,59 +493,75 @@ struct Builder {
         // Topological renumbering (src/database_cuda.hpp:2016-2080): a node is
         // pushed once all its child edges are visited; popping the stack gives
         // parents before children.
+        // Each node enters the order once, so the order and the walk's stack
+        // (node, next parent edge) each hold at most n entries.
         visit.assign(n, 0);
-        st.clear();
-        std::vector<int>& work = q;
+        st.resize(n);
+        q.resize(2 * (size_t)n);
+        int* order = st.data();
+        int* work = q.data();
+        const DNode* dnp = dn.data();
+        int* vis = visit.data();
+        int ns = 0;
         for (int i = 0; i < n; i++) {
-            if (child_num_pre[i] != 0) continue;
+            if (dnp[i].cpre != 0) continue;
             // iterative form of the recursive determinSequence
-            work.clear();
-            work.push_back(i);
-            work.push_back(0);
-            st.push_back(i);
-            while (!work.empty()) {
-                int idx = work[work.size() - 2];
-                int& k = work.back();
-                if (k == poff[idx + 1] - poff[idx]) {
-                    work.pop_back();
-                    work.pop_back();
+            int top = 0;
+            work[0] = i;
+            work[1] = poff[i];
+            order[ns++] = i;
+            while (top >= 0) {
+                const int idx = work[2 * top];
+                const int k = work[2 * top + 1];
+                if (k == poff[idx + 1]) {
+                    top--;
                     continue;
                 }
-                int p = plist[poff[idx] + k] / 10;
-                k++;
-                if (++visit[p] == child_num_pre[p]) {
-                    st.push_back(p);
-                    work.push_back(p);
-                    work.push_back(0);
+                work[2 * top + 1] = k + 1;
+                const int p = plist[k] / 10;
+                if (++vis[p] == dnp[p].cpre) {
+                    order[ns++] = p;
+                    top++;
+                    work[2 * top] = p;
+                    work[2 * top + 1] = poff[p];
                 }
             }
         }
-        const int64_t base_par = (int64_t)out.par.size();
+        st.resize(ns);
+        // Sized once and written through pointers; depth, npar and node are
+        // written for every node, child_num accumulates from zero.
         seq.assign(n, 0);
-        const size_t nbase = out.node.size();
+        const size_t nbase = out.node.size(), ebase = out.par.size();
         out.node.resize(nbase + n);
         out.child_num.resize(nbase + n, 0);
-        out.depth.resize(nbase + n, 0);
-        out.npar.resize(nbase + n, 0);
-        int copy = 0;
+        out.depth.resize(nbase + n);
+        out.npar.resize(nbase + n);
+        out.par.resize(ebase + edges.size());
+        out.slot.resize(ebase + edges.size());
+        int* o_node = out.node.data() + nbase;
+        int* o_cn = out.child_num.data() + nbase;
+        int* o_dep = out.depth.data() + nbase;
+        int* o_np = out.npar.data() + nbase;
+        int* o_par = out.par.data() + ebase;
+        int* o_slot = out.slot.data() + ebase;
+        int copy = 0, ne = 0;
         for (auto it = st.rbegin(); it != st.rend(); ++it) {
-            int idx = *it;
+            const int idx = *it;
             int dep = 0;
             for (int k = poff[idx]; k < poff[idx + 1]; k++) {
-                int p = plist[k] / 10, ci = plist[k] % 10;
-                int sp = seq[p];
-                out.child_num[nbase + sp] = std::max(out.child_num[nbase + sp], ci + 1);
-                out.par.push_back(sp);
-                out.slot.push_back(ci);
-                dep = std::max(dep, out.depth[nbase + sp] + 1);
+                const int p = plist[k] / 10, ci = plist[k] % 10;
+                const int sp = seq[p];
+                o_cn[sp] = std::max(o_cn[sp], ci + 1);
+                o_par[ne] = sp;
+                o_slot[ne++] = ci;
+                dep = std::max(dep, o_dep[sp] + 1);
             }
-            out.depth[nbase + copy] = dep;
-            out.npar[nbase + copy] = poff[idx + 1] - poff[idx];
+            o_dep[copy] = dep;
+            o_np[copy] = poff[idx + 1] - poff[idx];
             seq[idx] = copy;
-            out.node[nbase + copy++] = nodes[idx];
+            o_node[copy++] = dn[idx].cell;
         }
         if (copy != n) throw std::runtime_error("detour DAG has a cycle");
-        (void)base_par;
         out.nodes_per_net.push_back(n);
         out.points.insert(out.points.end(), points.begin(), points.end());
         out.points_per_net.push_back((int)points.size());
@@ -528,12 +577,12 @@ void build_dag(const Grid& g, const float* xsum, const float* ysum, int mode, co
     Builder b(g, xsum, ysum, mode);
     DagChunk c;
     b.build(tnode, tn, adj_off, tadj, c);
-    out.node = c.node;
-    out.child_num = c.child_num;
-    out.depth = c.depth;
-    out.par = c.par;
-    out.slot = c.slot;
-    out.points = c.points;
+    out.node.assign(c.node.begin(), c.node.end());
+    out.child_num.assign(c.child_num.begin(), c.child_num.end());
+    out.depth.assign(c.depth.begin(), c.depth.end());
+    out.par.assign(c.par.begin(), c.par.end());
+    out.slot.assign(c.slot.begin(), c.slot.end());
+    out.points.assign(c.points.begin(), c.points.end());
     out.par_off.assign(c.node.size() + 1, 0);
     for (size_t i = 0; i < c.node.size(); i++) out.par_off[i + 1] = out.par_off[i] + c.npar[i];
 }
@@ -561,7 +610,18 @@ void build_dags(const Grid& g, const Subnets& s, const int* nets, int64_t n, con
                         v->clear();
                     for (auto* v : {&ck.node, &ck.child_num, &ck.depth, &ck.npar}) v->reserve(CH * 64);
                     for (auto* v : {&ck.par, &ck.slot, &ck.points}) v->reserve(CH * 96);
-                    for (int64_t i = c * CH; i < std::min(n, (c + 1) * CH); i++) {
+                    const int64_t i1 = std::min(n, (c + 1) * CH);
+                    for (int64_t i = c * CH; i < i1; i++) {
+                        // The net list is not in subnet order: fetch the
+                        // trees of the next nets ahead, one level of the
+                        // offset chain per step.
+                        if (i + 3 < i1) __builtin_prefetch(&s.node_off[nets[i + 3]]);
+                        if (i + 2 < i1) {
+                            const int64_t bn = s.node_off[nets[i + 2]];
+                            __builtin_prefetch(&s.node[bn]);
+                            __builtin_prefetch(&s.adj_off[bn]);
+                        }
+                        if (i + 1 < i1) __builtin_prefetch(&s.adj[s.adj_off[s.node_off[nets[i + 1]]]]);
                         int net = nets[i];
                         int64_t b0 = s.node_off[net];
                         int tn = (int)(s.node_off[net + 1] - b0);

@@ -43,7 +43,7 @@ using RawVec = std::vector<T, NoInitAlloc<T>>;
 // Consecutive nets' DAGs as one builder thread produces them (par local,
 // npar = parent count per node).
 struct DagChunk {
-    std::vector<int> node, child_num, depth, npar, par, slot, points, nodes_per_net, points_per_net;
+    RawVec<int> node, child_num, depth, npar, par, slot, points, nodes_per_net, points_per_net;
 };
 
 // DAGs of several nets concatenated; par_off is global over all nodes and
