@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <atomic>
 #include <climits>
+#include <cstdint>
 #include <limits>
 #include <numeric>
 #include <stdexcept>
@@ -90,32 +91,52 @@ struct Dsu {
 void normalise(const Coords& C, const std::vector<int>& nx, const std::vector<int>& ny,
                const std::vector<std::pair<int, int>>& edges, Tree2D& t) {
     const int n = (int)nx.size();
-    std::vector<std::pair<int, int>> byPos(n);
-    for (int i = 0; i < n; i++) byPos[i] = {nx[i] * C.Y + ny[i], i};
+    // Positions are non-negative ints: (position, index) sorts as one
+    // unsigned 64-bit key.
+    std::vector<uint64_t> byPos(n);
+    for (int i = 0; i < n; i++) byPos[i] = (uint64_t)(uint32_t)(nx[i] * C.Y + ny[i]) << 32 | (uint32_t)i;
     std::sort(byPos.begin(), byPos.end());
     std::vector<int> id(n);
     t.pos.clear();
     for (int k = 0; k < n; k++) {
-        if (k == 0 || byPos[k].first != byPos[k - 1].first) t.pos.push_back(byPos[k].first);
-        id[byPos[k].second] = (int)t.pos.size() - 1;
+        if (k == 0 || byPos[k] >> 32 != byPos[k - 1] >> 32) t.pos.push_back((int)(byPos[k] >> 32));
+        id[(uint32_t)byPos[k]] = (int)t.pos.size() - 1;
     }
-    std::vector<std::tuple<long long, int, int>> es;
-    es.reserve(edges.size());
-    for (auto& e : edges) {
-        int u = id[e.first], v = id[e.second];
-        if (u == v) continue;
-        long long len = C.dist(nx[e.first], ny[e.first], nx[e.second], ny[e.second]);
-        es.emplace_back(len, std::min(u, v), std::max(u, v));
-    }
-    std::sort(es.begin(), es.end());
-    Dsu ds((int)t.pos.size());
     std::vector<std::pair<int, int>> acc;
-    acc.reserve(es.size());
-    for (auto& e : es) {
-        int u = std::get<1>(e), v = std::get<2>(e), pu = ds.find(u), pv = ds.find(v);
-        if (pu == pv) continue;
+    acc.reserve(edges.size());
+    Dsu ds((int)t.pos.size());
+    auto kruskal = [&](int u, int v) {
+        const int pu = ds.find(u), pv = ds.find(v);
+        if (pu == pv) return;
         acc.emplace_back(u, v);
         ds.p[pu] = pv;
+    };
+    // Edges by (length, u, v): one 64-bit key when the node ids fit 16 bits
+    // and the lengths 32, else the tuples themselves.
+    long long maxlen = 0;
+    for (auto& e : edges) maxlen = std::max(maxlen, C.dist(nx[e.first], ny[e.first], nx[e.second], ny[e.second]));
+    if (t.pos.size() <= 65536 && maxlen < (1ll << 32)) {
+        std::vector<uint64_t> es;
+        es.reserve(edges.size());
+        for (auto& e : edges) {
+            const int u = id[e.first], v = id[e.second];
+            if (u == v) continue;
+            const uint64_t len = (uint64_t)C.dist(nx[e.first], ny[e.first], nx[e.second], ny[e.second]);
+            es.push_back(len << 32 | (uint64_t)std::min(u, v) << 16 | (uint64_t)std::max(u, v));
+        }
+        std::sort(es.begin(), es.end());
+        for (uint64_t e : es) kruskal((int)(e >> 16 & 0xffff), (int)(e & 0xffff));
+    } else {
+        std::vector<std::tuple<long long, int, int>> es;
+        es.reserve(edges.size());
+        for (auto& e : edges) {
+            int u = id[e.first], v = id[e.second];
+            if (u == v) continue;
+            long long len = C.dist(nx[e.first], ny[e.first], nx[e.second], ny[e.second]);
+            es.emplace_back(len, std::min(u, v), std::max(u, v));
+        }
+        std::sort(es.begin(), es.end());
+        for (auto& e : es) kruskal(std::get<1>(e), std::get<2>(e));
     }
     adj_from_edges((int)t.pos.size(), acc, t.adj);
 }
@@ -489,8 +510,23 @@ struct SubOut {
 
 // Encode tree nodes with their pin layer or L (src/database_cuda.hpp:790-792).
 // The pins have distinct 2-D positions, so each node matches at most one.
+// Tree nodes come from normalise with ascending 2-D positions: then one merge
+// with the pins sorted by position finds every match.
 void encode_layers(const Grid& g, const std::vector<int>& pins, SubOut& s) {
     const int XY = g.X * g.Y;
+    if (std::is_sorted(s.node.begin(), s.node.end()) && (s.node.empty() || (s.node.front() >= 0 && s.node.back() < XY))) {
+        thread_local std::vector<std::pair<int, int>> byPos;
+        byPos.resize(pins.size());
+        for (size_t k = 0; k < pins.size(); k++) byPos[k] = {pins[k] % XY, pins[k] / XY};
+        std::sort(byPos.begin(), byPos.end());
+        size_t k = 0;
+        for (auto& e : s.node) {
+            while (k < byPos.size() && byPos[k].first < e) k++;
+            const int lay = k < byPos.size() && byPos[k].first == e ? byPos[k].second : g.L;
+            e += lay * XY;
+        }
+        return;
+    }
     if (pins.size() <= 16) {
         for (auto& e : s.node) {
             int pos = e % XY, lay = g.L;
