@@ -608,8 +608,9 @@ void build_dags(const Grid& g, const Subnets& s, const int* nets, int64_t n, con
                     for (auto* v : {&ck.node, &ck.child_num, &ck.depth, &ck.npar, &ck.par, &ck.slot, &ck.points,
                                     &ck.nodes_per_net, &ck.points_per_net})
                         v->clear();
-                    for (auto* v : {&ck.node, &ck.child_num, &ck.depth, &ck.npar}) v->reserve(CH * 64);
-                    for (auto* v : {&ck.par, &ck.slot, &ck.points}) v->reserve(CH * 96);
+                    for (auto* v : {&ck.node, &ck.child_num, &ck.depth, &ck.npar}) v->reserve(CH * 128);
+                    for (auto* v : {&ck.par, &ck.slot}) v->reserve(CH * 160);
+                    ck.points.reserve(CH * 96);
                     const int64_t i1 = std::min(n, (c + 1) * CH);
                     for (int64_t i = c * CH; i < i1; i++) {
                         // The net list is not in subnet order: fetch the
