@@ -123,6 +123,20 @@ typedef struct igr_dag_desc {
 int igr_dag_pass(igr_ctx* ctx, const int* nets, int64_t n, int mode, int form, const igr_dag_desc* dags,
                  igr_pass_stats* stats);
 
+/* Fills the DAG structure of a description whose n, pt_off and pts are set
+ * (node_off, nodes, par_off, par, slot, child_num, depth); returns IGR_OK or
+ * a non-zero status.  The arrays must stay valid until the pass returns. */
+typedef int (*igr_dag_complete_fn)(void* user, igr_dag_desc* dags);
+
+/* igr_dag_pass in the batched form with the DAGs' structure built while the
+ * scheduler runs: 'points' holds n, pt_off and pts (the scheduler's input);
+ * complete(user, d) is called once, on a helper thread, with a copy of it and
+ * fills the rest.  Routes are those of igr_dag_pass with the full
+ * description.  No counterpart in the reference, whose pass
+ * (src/Lshape_route_detour.hpp:726-1112) starts once the DAGs are built. */
+int igr_dag_pass_overlapped(igr_ctx* ctx, const int* nets, int64_t n, int mode, const igr_dag_desc* points,
+                            igr_dag_complete_fn complete, void* user, igr_pass_stats* stats);
+
 #define IGR_MAX_RRR 8
 
 typedef struct igr_run_opts {
@@ -141,6 +155,9 @@ typedef struct igr_run_stats {
     double rrr_view_s[IGR_MAX_RRR], rrr_dag_build_s[IGR_MAX_RRR], rrr_route_s[IGR_MAX_RRR];
     double rrr_schedule_s[IGR_MAX_RRR], rrr_prep_s[IGR_MAX_RRR];
     double rrr_prep_parts[IGR_MAX_RRR][4];   /* host fill, batch metadata, upload, allocation */
+    /* batched form: the DAG structure, built on the pass's helper thread
+     * while the scheduler runs (rrr_dag_build_s then times the points) */
+    double rrr_dag_rest_s[IGR_MAX_RRR];
 } igr_run_stats;
 
 struct igrh_design;

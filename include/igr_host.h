@@ -78,6 +78,19 @@ int64_t igrh_array(const igrh_design* h, const char* name, const void** ptr);
 int igrh_build_dags(igrh_design* h, const int* nets, int64_t n, const float* xsum, const float* ysum, int mode,
                     int threads);
 
+/* igrh_build_dags in two steps, so that the scheduler can start on the points
+ * while the rest is built (no counterpart in the reference, which builds the
+ * DAGs whole before its pass).  IGRH_DAG_POINTS writes only dag_pt_off and
+ * dag_pt; IGRH_DAG_STRUCTURE, called next with the same nets and views,
+ * writes the other dag_* arrays and leaves the points as they are (it fails
+ * unless the points of as many nets precede it); IGRH_DAG_ALL is
+ * igrh_build_dags.  The two steps give the arrays of one full build. */
+#define IGRH_DAG_ALL 0
+#define IGRH_DAG_POINTS 1
+#define IGRH_DAG_STRUCTURE 2
+int igrh_build_dags_part(igrh_design* h, const int* nets, int64_t n, const float* xsum, const float* ysum, int mode,
+                         int threads, int part);
+
 /* Rectilinear Steiner tree of n distinct gcells (x[i], y[i]) on a grid of
  * xsize*ysize gcells with edge lengths xlen (xsize-1 entries) and ylen
  * (ysize-1): the tree the subnet builder gives a net with these pins, in

@@ -529,22 +529,26 @@ void pfor(int64_t cnt, const std::function<void(int64_t, int64_t)>& fn) {
 // Batched form of one rip-up-and-reroute pass.  While the scheduler runs on
 // the compute stream, a helper thread uploads the DAGs as built (copy stream)
 // and measures every net (nodes, edges, slots, levels, algorithmic bytes).
+// With 'complete' set, dg holds only the points on entry and the helper first
+// has the rest of it built, so that building overlaps the scheduler too.
 // Once the routing order is known the host only computes per-net bases and
 // batch bounds; the device lays the DAGs out (k_dag_layout) and checks them.
-void dag_pass_batched(igr_ctx* c, const int* nets, int64_t n, int mode, const igr_dag_desc* dg, igr_pass_stats* st) {
+void dag_pass_batched(igr_ctx* c, const int* nets, int64_t n, int mode, const igr_dag_desc* dg_in,
+                      const std::function<void(igr_dag_desc&)>& complete, igr_pass_stats* st) {
     const int XY = c->g.X * c->g.Y, L = c->g.L, LL = L * L, ML = L + 1;
-    const int64_t NN = dg->node_off[n], NE = dg->par_off[NN];
+    igr_dag_desc dgv = *dg_in;
+    const igr_dag_desc* dg = &dgv;
     // The scheduler's points are checked here, the DAG structure on the device.
     std::atomic<int> bad_pt{0};
     pfor(n, [&](int64_t a0, int64_t a1) {
         int bad = 0;
         for (int64_t i = a0; i < a1; i++) {
-            bad |= dg->node_off[i + 1] <= dg->node_off[i];
+            bad |= dg->pt_off[i + 1] < dg->pt_off[i];
             for (int64_t q = dg->pt_off[i]; q < dg->pt_off[i + 1]; q++) bad |= dg->pts[q] < 0 || dg->pts[q] >= XY;
         }
         if (bad) bad_pt.store(1);
     });
-    if (bad_pt.load()) throw std::invalid_argument("empty DAG or DAG point outside the grid");
+    if (bad_pt.load()) throw std::invalid_argument("DAG point outside the grid");
     DagDev& D = c->dag;
     hipStream_t s = c->stream;
     const double t0 = now_s();
@@ -557,7 +561,7 @@ void dag_pass_batched(igr_ctx* c, const int* nets, int64_t n, int mode, const ig
             pairs[layer_dir(fl, c->g.DIR)] += 1;
             pvia[layer_dir(fl, c->g.DIR)] += std::abs(fl - tl);
         }
-    if (dg->node_off[0] != 0 || dg->par_off[0] != 0) throw std::invalid_argument("DAG offsets must start at 0");
+    int64_t NN = 0, NE = 0;
     std::vector<int64_t> l_ns(n);
     std::vector<int> l_nl(n);
     std::vector<double> l_eb(n), l_db(n);
@@ -567,6 +571,16 @@ void dag_pass_batched(igr_ctx* c, const int* nets, int64_t n, int mode, const ig
     std::thread helper([&]() {
         try {
             const double h0 = now_s();
+            if (complete) complete(dgv);
+            if (dg->node_off[0] != 0 || dg->par_off[0] != 0) throw std::invalid_argument("DAG offsets must start at 0");
+            std::atomic<int> empty{0};
+            pfor(n, [&](int64_t a0, int64_t a1) {
+                for (int64_t i = a0; i < a1; i++)
+                    if (dg->node_off[i + 1] <= dg->node_off[i]) empty.store(1);
+            });
+            if (empty.load()) throw std::invalid_argument("empty DAG");
+            NN = dg->node_off[n];
+            NE = dg->par_off[NN];
             hipStream_t cs = c->copy_stream;
             D.r_node_off.upload(dg->node_off, n + 1, cs);
             D.r_nodes.upload(dg->nodes, NN, cs);
@@ -830,7 +844,7 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
             return;
         }
         if (form == IGR_FORM_BATCHED) {
-            dag_pass_batched(c, nets, n, mode, dg, st);
+            dag_pass_batched(c, nets, n, mode, dg, nullptr, st);
             return;
         }
         const int XY = c->g.X * c->g.Y, L = c->g.L, LL = L * L, ML = L + 1;
@@ -1066,6 +1080,30 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
         c->last_dag_prep_parts[2] = t_up - t_meta;
         c->last_dag_prep_parts[3] = tp - t_up;
         if (st) *st = igr_pass_stats{nb, t1 - t0, t2 - t1, n};
+    });
+}
+
+int igr_dag_pass_overlapped(igr_ctx* c, const int* nets, int64_t n, int mode, const igr_dag_desc* points,
+                            igr_dag_complete_fn complete, void* user, igr_pass_stats* st) {
+    return guard([&]() {
+        check_nets(c, nets, n);
+        if (!points || points->n != n || !points->pt_off || !points->pts)
+            throw std::invalid_argument("one point list per listed net");
+        if (!complete) throw std::invalid_argument("no completion callback");
+        if (n == 0) {
+            if (st) *st = igr_pass_stats{0, 0, 0, 0};
+            return;
+        }
+        dag_pass_batched(c, nets, n, mode, points, [&](igr_dag_desc& d) {
+            const int rc = complete(user, &d);
+            if (rc != IGR_OK) throw std::invalid_argument("DAG completion failed with status " + std::to_string(rc));
+            // the points are the ones the scheduler already has
+            d.n = n;
+            d.pt_off = points->pt_off;
+            d.pts = points->pts;
+            if (!d.node_off || !d.nodes || !d.par_off || !d.par || !d.slot || !d.child_num || !d.depth)
+                throw std::invalid_argument("DAG completion left arrays unset");
+        }, st);
     });
 }
 

@@ -82,14 +82,61 @@ int igr_run(igr_ctx* c, igrh_design* h, const igr_run_opts* o, igr_run_stats* st
         S.ripped[it] = (int64_t)nets.size();
         if (nets.empty()) break;
         double b = now_s();
-        if ((rc = igrh_build_dags(h, nets.data(), (int64_t)nets.size(), xs.data(), ys.data(), mode, threads))) {
-            return fail(IGR_EINVAL, std::string("detour DAGs: ") + igrh_error());
-        }
-        double cc = now_s();
         const igr::DagSet& D = *igrh_dags_ref(h);
-        igr_dag_desc dd{D.n, D.node_off.data(), D.node.data(), D.par_off.data(), D.par.data(), D.slot.data(),
-                        D.child_num.data(), D.depth.data(), D.pt_off.data(), D.pt.data()};
-        if ((rc = igr_dag_pass(c, nets.data(), (int64_t)nets.size(), mode, form, &dd, &ps))) return rc;
+        double cc;
+        if (form == IGR_FORM_BATCHED) {
+            // The scheduler needs only the DAGs' points: build those, then
+            // the rest of each DAG on the pass's helper thread while the
+            // scheduler runs, with three host threads left to the scheduler's
+            // launches and the uploads.
+            if ((rc = igrh_build_dags_part(h, nets.data(), (int64_t)nets.size(), xs.data(), ys.data(), mode, threads,
+                                           IGRH_DAG_POINTS))) {
+                return fail(IGR_EINVAL, std::string("detour DAG points: ") + igrh_error());
+            }
+            cc = now_s();
+            struct Completion {
+                igrh_design* h;
+                const std::vector<int>* nets;
+                const float *xs, *ys;
+                int mode, threads;
+                double secs;
+                std::string err;
+            } cp{h, &nets, xs.data(), ys.data(), mode, std::max(1, threads - 3), 0.0, std::string()};
+            auto complete = [](void* user, igr_dag_desc* d) -> int {
+                Completion& x = *static_cast<Completion*>(user);
+                const double t = now_s();
+                if (igrh_build_dags_part(x.h, x.nets->data(), (int64_t)x.nets->size(), x.xs, x.ys, x.mode, x.threads,
+                                         IGRH_DAG_STRUCTURE)) {
+                    x.err = igrh_error();
+                    return IGR_EINVAL;
+                }
+                const igr::DagSet& S = *igrh_dags_ref(x.h);
+                d->node_off = S.node_off.data();
+                d->nodes = S.node.data();
+                d->par_off = S.par_off.data();
+                d->par = S.par.data();
+                d->slot = S.slot.data();
+                d->child_num = S.child_num.data();
+                d->depth = S.depth.data();
+                x.secs = now_s() - t;
+                return IGR_OK;
+            };
+            igr_dag_desc pd{D.n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, D.pt_off.data(),
+                            D.pt.data()};
+            if ((rc = igr_dag_pass_overlapped(c, nets.data(), (int64_t)nets.size(), mode, &pd, complete, &cp, &ps))) {
+                if (!cp.err.empty()) return fail(rc, std::string("detour DAGs: ") + cp.err);
+                return rc;
+            }
+            S.rrr_dag_rest_s[it] = cp.secs;
+        } else {
+            if ((rc = igrh_build_dags(h, nets.data(), (int64_t)nets.size(), xs.data(), ys.data(), mode, threads))) {
+                return fail(IGR_EINVAL, std::string("detour DAGs: ") + igrh_error());
+            }
+            cc = now_s();
+            igr_dag_desc dd{D.n, D.node_off.data(), D.node.data(), D.par_off.data(), D.par.data(), D.slot.data(),
+                            D.child_num.data(), D.depth.data(), D.pt_off.data(), D.pt.data()};
+            if ((rc = igr_dag_pass(c, nets.data(), (int64_t)nets.size(), mode, form, &dd, &ps))) return rc;
+        }
         double e = now_s();
         S.rrr_view_s[it] = b - a;
         S.rrr_dag_build_s[it] = cc - b;
@@ -100,8 +147,8 @@ int igr_run(igr_ctx* c, igrh_design* h, const igr_run_opts* o, igr_run_stats* st
         S.rrr_batches[it] = ps.batches;
         S.rrr_dag_nodes[it] = D.node_off.empty() ? 0 : D.node_off.back();
         S.iterations = it + 1;
-        progress("rip-up-and-reroute pass %d: %.3f s (view %.3f, DAG build %.3f, %lld nets)", it, e - a, b - a,
-                 cc - b, (long long)nets.size());
+        progress("rip-up-and-reroute pass %d: %.3f s (view %.3f, DAG build %.3f, behind the scheduler %.3f, %lld nets)",
+                 it, e - a, b - a, cc - b, S.rrr_dag_rest_s[it], (long long)nets.size());
     }
     S.total_s = now_s() - t0;
     S.subnets = pi.n;

@@ -441,12 +441,15 @@ struct Builder {
     }
 
     // Build one net's DAG and append it to the chunk output.
-    void build(const int* tnode, int tn_, const int64_t* adj_off_abs, const int* tadj_, DagChunk& out) {
+    void build(const int* tnode, int tn_, const int64_t* adj_off_abs, const int* tadj_, DagChunk& out,
+               int part = DAG_ALL) {
         // The chunk arrays were last written a whole build ago: their next
         // lines (a typical net's share) load while the DAG is being built.
-        for (auto* v : {&out.node, &out.child_num, &out.depth, &out.npar}) prefetch_tail(*v, 384);
-        for (auto* v : {&out.par, &out.slot}) prefetch_tail(*v, 512);
-        prefetch_tail(out.points, 256);
+        if (part != DAG_POINTS) {
+            for (auto* v : {&out.node, &out.child_num, &out.depth, &out.npar}) prefetch_tail(*v, 384);
+            for (auto* v : {&out.par, &out.slot}) prefetch_tail(*v, 512);
+        }
+        if (part != DAG_STRUCTURE) prefetch_tail(out.points, 256);
         tn = tn_;
         pos = tnode;
         tadj = tadj_;
@@ -479,7 +482,12 @@ struct Builder {
         edges.clear();
         points.clear();
         dfs(root, -1, -1, -1, 0, 0);
-        dedup_points();
+        if (part != DAG_STRUCTURE) {
+            dedup_points();
+            out.points.insert(out.points.end(), points.begin(), points.end());
+            out.points_per_net.push_back((int)points.size());
+            if (part == DAG_POINTS) return;
+        }
 
         // Parent lists in creation order (CSR by child).
         const int n = (int)dn.size();
@@ -563,8 +571,6 @@ struct Builder {
         }
         if (copy != n) throw std::runtime_error("detour DAG has a cycle");
         out.nodes_per_net.push_back(n);
-        out.points.insert(out.points.end(), points.begin(), points.end());
-        out.points_per_net.push_back((int)points.size());
     }
 
     std::vector<int64_t> rel;
@@ -588,7 +594,11 @@ void build_dag(const Grid& g, const float* xsum, const float* ysum, int mode, co
 }
 
 void build_dags(const Grid& g, const Subnets& s, const int* nets, int64_t n, const float* xsum, const float* ysum,
-                int mode, int threads, DagSet& out) {
+                int mode, int threads, DagSet& out, int part) {
+    if (part != DAG_ALL && part != DAG_POINTS && part != DAG_STRUCTURE) throw std::invalid_argument("unknown DAG part");
+    const bool want_pts = part != DAG_STRUCTURE, want_dag = part != DAG_POINTS;
+    if (part == DAG_STRUCTURE && (out.n != n || (int64_t)out.pt_off.size() != n + 1))
+        throw std::invalid_argument("a structure build follows a points build of the same nets");
     if (threads < 1) threads = 1;
     const int64_t CH = 512;
     const int64_t nch = (n + CH - 1) / CH;
@@ -608,9 +618,11 @@ void build_dags(const Grid& g, const Subnets& s, const int* nets, int64_t n, con
                     for (auto* v : {&ck.node, &ck.child_num, &ck.depth, &ck.npar, &ck.par, &ck.slot, &ck.points,
                                     &ck.nodes_per_net, &ck.points_per_net})
                         v->clear();
-                    for (auto* v : {&ck.node, &ck.child_num, &ck.depth, &ck.npar}) v->reserve(CH * 128);
-                    for (auto* v : {&ck.par, &ck.slot}) v->reserve(CH * 160);
-                    ck.points.reserve(CH * 96);
+                    if (want_dag) {
+                        for (auto* v : {&ck.node, &ck.child_num, &ck.depth, &ck.npar}) v->reserve(CH * 128);
+                        for (auto* v : {&ck.par, &ck.slot}) v->reserve(CH * 160);
+                    }
+                    if (want_pts) ck.points.reserve(CH * 96);
                     const int64_t i1 = std::min(n, (c + 1) * CH);
                     for (int64_t i = c * CH; i < i1; i++) {
                         // The net list is not in subnet order: fetch the
@@ -626,7 +638,7 @@ void build_dags(const Grid& g, const Subnets& s, const int* nets, int64_t n, con
                         int net = nets[i];
                         int64_t b0 = s.node_off[net];
                         int tn = (int)(s.node_off[net + 1] - b0);
-                        b.build(&s.node[b0], tn, &s.adj_off[b0], &s.adj[s.adj_off[b0]], ck);
+                        b.build(&s.node[b0], tn, &s.adj_off[b0], &s.adj[s.adj_off[b0]], ck, part);
                     }
                 }
             } catch (const std::exception& e) {
@@ -651,15 +663,19 @@ void build_dags(const Grid& g, const Subnets& s, const int* nets, int64_t n, con
         if ((int64_t)v.capacity() < m) v.reserve((size_t)(m + m / 4));
         v.resize((size_t)m);
     };
-    size_to(out.node_off, n + 1);
-    size_to(out.pt_off, n + 1);
-    size_to(out.node, cn[nch]);
-    size_to(out.child_num, cn[nch]);
-    size_to(out.depth, cn[nch]);
-    size_to(out.par_off, cn[nch] + 1);
-    size_to(out.par, ce[nch]);
-    size_to(out.slot, ce[nch]);
-    size_to(out.pt, cp[nch]);
+    if (want_dag) {
+        size_to(out.node_off, n + 1);
+        size_to(out.node, cn[nch]);
+        size_to(out.child_num, cn[nch]);
+        size_to(out.depth, cn[nch]);
+        size_to(out.par_off, cn[nch] + 1);
+        size_to(out.par, ce[nch]);
+        size_to(out.slot, ce[nch]);
+    }
+    if (want_pts) {
+        size_to(out.pt_off, n + 1);
+        size_to(out.pt, cp[nch]);
+    }
     next = 0;
     pool.clear();
     for (int t = 0; t < threads; t++)
@@ -668,31 +684,39 @@ void build_dags(const Grid& g, const Subnets& s, const int* nets, int64_t n, con
                 int64_t c = next.fetch_add(1);
                 if (c >= nch) break;
                 const DagChunk& ck = chunks[c];
-                std::copy(ck.node.begin(), ck.node.end(), out.node.begin() + cn[c]);
-                std::copy(ck.child_num.begin(), ck.child_num.end(), out.child_num.begin() + cn[c]);
-                std::copy(ck.depth.begin(), ck.depth.end(), out.depth.begin() + cn[c]);
-                std::copy(ck.par.begin(), ck.par.end(), out.par.begin() + ce[c]);
-                std::copy(ck.slot.begin(), ck.slot.end(), out.slot.begin() + ce[c]);
-                std::copy(ck.points.begin(), ck.points.end(), out.pt.begin() + cp[c]);
-                int64_t e = ce[c];
-                for (size_t k = 0; k < ck.npar.size(); k++) {
-                    out.par_off[cn[c] + k] = e;
-                    e += ck.npar[k];
+                if (want_dag) {
+                    std::copy(ck.node.begin(), ck.node.end(), out.node.begin() + cn[c]);
+                    std::copy(ck.child_num.begin(), ck.child_num.end(), out.child_num.begin() + cn[c]);
+                    std::copy(ck.depth.begin(), ck.depth.end(), out.depth.begin() + cn[c]);
+                    std::copy(ck.par.begin(), ck.par.end(), out.par.begin() + ce[c]);
+                    std::copy(ck.slot.begin(), ck.slot.end(), out.slot.begin() + ce[c]);
+                    int64_t e = ce[c];
+                    for (size_t k = 0; k < ck.npar.size(); k++) {
+                        out.par_off[cn[c] + k] = e;
+                        e += ck.npar[k];
+                    }
+                    int64_t nn = cn[c];
+                    for (size_t k = 0; k < ck.nodes_per_net.size(); k++) {
+                        out.node_off[c * CH + (int64_t)k] = nn;
+                        nn += ck.nodes_per_net[k];
+                    }
                 }
-                int64_t nn = cn[c], pp = cp[c];
-                for (size_t k = 0; k < ck.nodes_per_net.size(); k++) {
-                    int64_t i = c * CH + (int64_t)k;
-                    out.node_off[i] = nn;
-                    out.pt_off[i] = pp;
-                    nn += ck.nodes_per_net[k];
-                    pp += ck.points_per_net[k];
+                if (want_pts) {
+                    std::copy(ck.points.begin(), ck.points.end(), out.pt.begin() + cp[c]);
+                    int64_t pp = cp[c];
+                    for (size_t k = 0; k < ck.points_per_net.size(); k++) {
+                        out.pt_off[c * CH + (int64_t)k] = pp;
+                        pp += ck.points_per_net[k];
+                    }
                 }
             }
         });
     for (auto& th : pool) th.join();
-    out.par_off[cn[nch]] = ce[nch];
-    out.node_off[n] = cn[nch];
-    out.pt_off[n] = cp[nch];
+    if (want_dag) {
+        out.par_off[cn[nch]] = ce[nch];
+        out.node_off[n] = cn[nch];
+    }
+    if (want_pts) out.pt_off[n] = cp[nch];
 }
 
 } // namespace igr

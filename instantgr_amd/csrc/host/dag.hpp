@@ -59,7 +59,13 @@ struct DagSet {
 void build_dag(const Grid& g, const float* xsum, const float* ysum, int mode, const int* tnode, int tn,
                const int64_t* adj_off, const int* tadj, DagNet& out);
 
+// What a build writes into the DAG set: every array, only the scheduler
+// points (pt_off, pt), or every array but the points, which it leaves as they
+// are.  A points build followed by a structure build over the same nets and
+// views gives the arrays of a full build.
+enum DagPart { DAG_ALL = 0, DAG_POINTS = 1, DAG_STRUCTURE = 2 };
+
 void build_dags(const Grid& g, const Subnets& s, const int* nets, int64_t n, const float* xsum, const float* ysum,
-                int mode, int threads, DagSet& out);
+                int mode, int threads, DagSet& out, int part = DAG_ALL);
 
 } // namespace igr

@@ -141,14 +141,19 @@ int64_t igrh_array(const igrh_design* h, const char* name, const void** ptr) {
     return -1;
 }
 
-int igrh_build_dags(igrh_design* h, const int* nets, int64_t n, const float* xsum, const float* ysum, int mode,
-                     int threads) {
+int igrh_build_dags_part(igrh_design* h, const int* nets, int64_t n, const float* xsum, const float* ysum, int mode,
+                         int threads, int part) {
     return guard([&]() {
         if (!h->built) throw std::runtime_error("design not built");
         for (int64_t i = 0; i < n; i++)
             if (nets[i] < 0 || nets[i] >= h->s.count) throw std::runtime_error("net id out of range");
-        build_dags(h->g, h->s, nets, n, xsum, ysum, mode, threads, h->dags);
+        build_dags(h->g, h->s, nets, n, xsum, ysum, mode, threads, h->dags, part);
     });
+}
+
+int igrh_build_dags(igrh_design* h, const int* nets, int64_t n, const float* xsum, const float* ysum, int mode,
+                     int threads) {
+    return igrh_build_dags_part(h, nets, n, xsum, ysum, mode, threads, IGRH_DAG_ALL);
 }
 
 int igrh_write_routes(const igrh_design* h, const char* out_path, const int64_t* off, const int* data,

@@ -88,7 +88,7 @@ class RunStats(C.Structure):
                 ("rrr_dag_nodes", C.c_int64 * MAX_RRR), ("rrr_view_s", C.c_double * MAX_RRR),
                 ("rrr_dag_build_s", C.c_double * MAX_RRR), ("rrr_route_s", C.c_double * MAX_RRR),
                 ("rrr_schedule_s", C.c_double * MAX_RRR), ("rrr_prep_s", C.c_double * MAX_RRR),
-                ("rrr_prep_parts", (C.c_double * 4) * MAX_RRR)]
+                ("rrr_prep_parts", (C.c_double * 4) * MAX_RRR), ("rrr_dag_rest_s", C.c_double * MAX_RRR)]
 
 
 # Every entry point of include/igr.h with its ctypes signature.
@@ -109,6 +109,8 @@ SIGNATURES = {
     "igr_pattern_route": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.POINTER(PassStats)]),
     "igr_dag_pass": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.POINTER(DagDesc),
                                C.POINTER(PassStats)]),
+    "igr_dag_pass_overlapped": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.POINTER(DagDesc),
+                                          C.c_void_p, C.c_void_p, C.POINTER(PassStats)]),
     "igr_get_route": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "igr_get_routes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
     "igr_evaluate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -307,6 +309,51 @@ class Router:
         _check(lib().igr_dag_pass(self.h, nets.ctypes.data, len(nets), mode, form, C.byref(d), C.byref(st)))
         return {"batches": st.batches, "schedule_s": st.schedule_s, "route_s": st.route_s, "nets": st.nets}
 
+    def dag_pass_overlapped(self, nets, points, complete, mode=0):
+        """igr_dag_pass_overlapped (batched form): 'points' holds dag_pt_off and
+        dag_pt; complete() is called once, on the pass's helper thread while
+        the scheduler runs, and returns the other DAG arrays (the dict of
+        Design.build_dags with part DAG_STRUCTURE).  An exception in it fails
+        the pass."""
+        nets = np.ascontiguousarray(nets, np.int32)
+        pt_off = np.ascontiguousarray(points["dag_pt_off"], np.int64)
+        pt = np.ascontiguousarray(points["dag_pt"], np.int32)
+        n = len(nets)
+        if len(pt_off) != n + 1 or len(pt) < pt_off[-1]:
+            raise ValueError("dag_pt_off needs one entry per net plus one, dag_pt as many points as it says")
+        keys = (("dag_node_off", np.int64), ("dag_node", np.int32), ("dag_par_off", np.int64),
+                ("dag_par", np.int32), ("dag_slot", np.int32), ("dag_child_num", np.int32),
+                ("dag_depth", np.int32))
+        held = []
+        errors = []
+
+        def fill(_user, desc):
+            try:
+                got = complete()
+                arrs = [np.ascontiguousarray(got[k], dt) for k, dt in keys]
+                node_off, node, par_off, par, slot, cnum, depth = arrs
+                nodes = int(node_off[-1])
+                if len(node_off) != n + 1 or len(par_off) != nodes + 1 or len(node) < nodes or \
+                        len(cnum) < nodes or len(depth) < nodes or len(par) < par_off[-1] or len(slot) < par_off[-1]:
+                    raise ValueError("DAG arrays shorter than their offsets say")
+                held.extend(arrs)
+                d = desc.contents
+                (d.node_off, d.nodes, d.par_off, d.par, d.slot, d.child_num, d.depth) = [a.ctypes.data for a in arrs]
+                return 0
+            except Exception as e:  # reported after the call; the pass fails
+                errors.append(e)
+                return -1
+
+        cb = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(DagDesc))(fill)
+        d = DagDesc(n, None, None, None, None, None, None, None, pt_off.ctypes.data, pt.ctypes.data)
+        st = PassStats()
+        rc = lib().igr_dag_pass_overlapped(self.h, nets.ctypes.data, n, mode, C.byref(d),
+                                           C.cast(cb, C.c_void_p), None, C.byref(st))
+        if errors:
+            raise errors[0]
+        _check(rc)
+        return {"batches": st.batches, "schedule_s": st.schedule_s, "route_s": st.route_s, "nets": st.nets}
+
     def route(self, net):
         buf = np.empty(65536, np.int32)
         ln = C.c_int()
@@ -423,7 +470,8 @@ class Router:
                 "rrr_dag_nodes": list(st.rrr_dag_nodes[:it]), "rrr_view_s": list(st.rrr_view_s[:it]),
                 "rrr_dag_build_s": list(st.rrr_dag_build_s[:it]), "rrr_route_s": list(st.rrr_route_s[:it]),
                 "rrr_schedule_s": list(st.rrr_schedule_s[:it]), "rrr_prep_s": list(st.rrr_prep_s[:it]),
-                "rrr_prep_parts": [list(st.rrr_prep_parts[k]) for k in range(it)]}
+                "rrr_prep_parts": [list(st.rrr_prep_parts[k]) for k in range(it)],
+                "rrr_dag_rest_s": list(st.rrr_dag_rest_s[:it])}
 
     @property
     def stamp(self):

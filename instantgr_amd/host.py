@@ -8,6 +8,11 @@ import numpy as np
 _LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
 _lib = None
 
+# igrh_build_dags_part parts (include/igr_host.h) and the arrays each writes
+DAG_ALL, DAG_POINTS, DAG_STRUCTURE = 0, 1, 2
+DAG_POINT_KEYS = ("dag_pt_off", "dag_pt")
+DAG_STRUCTURE_KEYS = ("dag_node_off", "dag_node", "dag_par_off", "dag_par", "dag_slot", "dag_child_num", "dag_depth")
+
 
 class GenParams(C.Structure):
     _fields_ = [("layers", C.c_int), ("xsize", C.c_int), ("ysize", C.c_int), ("nets", C.c_int64),
@@ -54,6 +59,8 @@ def lib():
         L.igrh_array.restype = C.c_int64
         L.igrh_write_routes.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p, C.c_int64]
         L.igrh_build_dags.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+        L.igrh_build_dags_part.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                           C.c_int]
         L.igrh_evaluate.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(Score)]
         _lib = L
     return _lib
@@ -140,16 +147,20 @@ class Design:
         ylen = np.ctypeslib.as_array(i.ylen, shape=(i.Y - 1,))
         return cap, w, xlen, ylen
 
-    def build_dags(self, nets, xsum, ysum, mode=0, threads=8):
+    def build_dags(self, nets, xsum, ysum, mode=0, threads=8, part=DAG_ALL):
+        """igrh_build_dags_part: the arrays the part writes (DAG_POINTS: dag_pt_off
+        and dag_pt; DAG_STRUCTURE, after DAG_POINTS on the same nets: the
+        others; DAG_ALL: all nine)."""
         nets = np.ascontiguousarray(nets, np.int32)
         xsum = np.ascontiguousarray(xsum, np.float32)
         ysum = np.ascontiguousarray(ysum, np.float32)
-        _check(lib().igrh_build_dags(self.h, nets.ctypes.data, len(nets), xsum.ctypes.data, ysum.ctypes.data,
-                                     mode, threads))
+        _check(lib().igrh_build_dags_part(self.h, nets.ctypes.data, len(nets), xsum.ctypes.data, ysum.ctypes.data,
+                                          mode, threads, part))
         for k in [k for k in self._arrays if k.startswith("dag_")]:
             del self._arrays[k]
-        return {k: self.array(k).copy() for k in ("dag_node_off", "dag_node", "dag_par_off", "dag_par", "dag_slot",
-                                                  "dag_child_num", "dag_depth", "dag_pt_off", "dag_pt")}
+        keys = {DAG_ALL: DAG_POINT_KEYS + DAG_STRUCTURE_KEYS, DAG_POINTS: DAG_POINT_KEYS,
+                DAG_STRUCTURE: DAG_STRUCTURE_KEYS}[part]
+        return {k: self.array(k).copy() for k in keys}
 
     def write_routes(self, path, off, data):
         off = np.ascontiguousarray(off, dtype=np.int64)

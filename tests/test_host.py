@@ -250,6 +250,37 @@ def test_dag_builder_invariants(small_design):
             assert {int(v) % XY for v in node[b:e]} <= set(pt[ptoff[k]:ptoff[k + 1]].tolist())
 
 
+def test_dag_build_in_two_steps_equals_one(small_design):
+    """The points build and the structure build after it give the arrays of
+    one full build, in every mode; a structure build over a different net
+    count than the points before it is refused."""
+    from instantgr_amd.host import DAG_ALL, DAG_POINTS, DAG_STRUCTURE, DAG_POINT_KEYS, DAG_STRUCTURE_KEYS
+    d = small_design
+    o = orc.Oracle.from_design(d)
+    o.set_nets_from_design(d)
+    o.pattern_route(d.array("order"))
+    xs, ys = o.congestion_view()
+    hp = d.array("hpwl")
+    nets = np.arange(d.info.subnets, dtype=np.int32)
+    nets = nets[np.argsort(-hp[nets], kind="stable")]
+    for mode in (0, 1, 2):
+        whole = d.build_dags(nets, xs, ys, mode, 4, DAG_ALL)
+        pts = d.build_dags(nets, xs, ys, mode, 3, DAG_POINTS)
+        assert set(pts) == set(DAG_POINT_KEYS)
+        rest = d.build_dags(nets, xs, ys, mode, 2, DAG_STRUCTURE)
+        assert set(rest) == set(DAG_STRUCTURE_KEYS)
+        for k in DAG_POINT_KEYS:
+            assert np.array_equal(pts[k], whole[k]), (mode, k)
+        for k in DAG_STRUCTURE_KEYS:
+            assert np.array_equal(rest[k], whole[k]), (mode, k)
+        # the structure build leaves the points as they are
+        assert np.array_equal(d.array("dag_pt"), whole["dag_pt"])
+    with pytest.raises(RuntimeError):
+        d.build_dags(nets[:-1], xs, ys, 0, 2, DAG_STRUCTURE)
+    with pytest.raises(RuntimeError):
+        d.build_dags(nets, xs, ys, 0, 2, 7)
+
+
 # ---------------------------------------------------------------------------
 # C ABI and command line
 # ---------------------------------------------------------------------------
