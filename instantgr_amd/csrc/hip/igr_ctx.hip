@@ -1,6 +1,7 @@
 // C ABI of libigr.so: context lifetime, grid state transfers, nets, the
 // scheduler and the pattern pass.
 #include "igr_ctx.hpp"
+#include "par.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -185,55 +186,80 @@ int igr_compute_presum(igr_ctx* c) {
     });
 }
 
+namespace {
+// Every route starts empty: r[0] = 1 at each net's slot (the rest is zero).
+__global__ void k_route_starts(const int64_t* route_off, int64_t n, int* routes) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) routes[route_off[i]] = 1;
+}
+
+int host_threads() { return std::max(1, std::min(16, (int)std::thread::hardware_concurrency())); }
+} // namespace
+
 int igr_set_nets(igr_ctx* c, const igr_net_desc* d) {
     return guard([&]() {
         if (!d || d->n < 0 || !d->node_off || !d->nodes || !d->parents)
             throw std::invalid_argument("net descriptor needs node_off, nodes and parents");
         NetsDev& N = c->nets;
         const int64_t n = d->n;
-        const int XY = c->g.X * c->g.Y;
+        const int XY = c->g.X * c->g.Y, Y = c->g.Y, Lr = c->g.L;
+        const int64_t ncells = (int64_t)(c->g.L + 1) * XY;
+        const int nth = host_threads();
         N.n = n;
         N.h_node_off.assign(d->node_off, d->node_off + n + 1);
         N.h_node.assign(d->nodes, d->nodes + N.h_node_off[n]);
         N.h_route_off.assign(n + 1, 0);
-        N.max_nodes = 0;
-        for (int64_t i = 0; i < n; i++) {
-            int64_t k = N.h_node_off[i + 1] - N.h_node_off[i];
-            if (k < 1) throw std::invalid_argument("every net needs at least one node");
-            for (int64_t j = 0; j < k; j++) {
-                int p = d->parents[N.h_node_off[i] + j];
-                if (j == 0 ? p != -1 : (p < 0 || p >= j)) throw std::invalid_argument("nodes must be in DFS pre-order");
-                int v = d->nodes[N.h_node_off[i] + j];
-                if (v < 0 || v >= (c->g.L + 1) * XY) throw std::invalid_argument("node outside the grid");
-            }
-            N.max_nodes = std::max<int>(N.max_nodes, (int)k);
-            N.h_route_off[i + 1] = N.h_route_off[i] + std::max<int64_t>(3, 8 * k - 5);
-        }
-        // Algorithmic bytes of each net's L-shape DP: per node its position,
-        // parent, via-stack costs (4 B per layer) and route output; per
-        // straight edge two prefix sums per layer; per L-shaped edge four
-        // prefix sums per layer plus the corner via costs.
         N.h_lbytes.assign(n, 0.0);
-        const int Lr = c->g.L;
-        for (int64_t i = 0; i < n; i++) {
-            const int64_t b = N.h_node_off[i], k = N.h_node_off[i + 1] - b;
-            int64_t straight = 0;
-            for (int64_t j = 1; j < k; j++) {
-                const int v = d->nodes[b + j], u = d->nodes[b + d->parents[b + j]];
-                if ((v % XY) / c->g.Y == (u % XY) / c->g.Y || v % c->g.Y == u % c->g.Y) straight++;
+        // Per net, in parallel: the DFS pre-order and grid checks, its route
+        // slot size (prefix-summed below) and the algorithmic bytes of its
+        // L-shape DP: per node its position, parent, via-stack costs (4 B per
+        // layer) and route output; per straight edge two prefix sums per
+        // layer; per L-shaped edge four prefix sums per layer plus the corner
+        // via costs.
+        std::vector<int> mxn(nth, 0);
+        igr::parallel_ranges(n, nth, [&](int64_t a, int64_t b, int t) {
+            for (int64_t i = a; i < b; i++) {
+                const int64_t o = N.h_node_off[i], k = N.h_node_off[i + 1] - o;
+                if (k < 1) throw std::invalid_argument("every net needs at least one node");
+                int64_t straight = 0;
+                for (int64_t j = 0; j < k; j++) {
+                    const int p = d->parents[o + j];
+                    if (j == 0 ? p != -1 : (p < 0 || p >= j)) throw std::invalid_argument("nodes must be in DFS pre-order");
+                    const int v = d->nodes[o + j];
+                    if (v < 0 || v >= ncells) throw std::invalid_argument("node outside the grid");
+                    if (j > 0) {
+                        const int u = d->nodes[o + p];
+                        if ((v % XY) / Y == (u % XY) / Y || v % Y == u % Y) straight++;
+                    }
+                }
+                mxn[t] = std::max<int>(mxn[t], (int)k);
+                N.h_route_off[i + 1] = std::max<int64_t>(3, 8 * k - 5);
+                N.h_lbytes[i] = (double)k * (8 + 4 * Lr + 32) + (double)straight * 8 * Lr +
+                                (double)(k - 1 - straight) * 40 * Lr;
             }
-            N.h_lbytes[i] = (double)k * (8 + 4 * Lr + 32) + (double)straight * 8 * Lr +
-                            (double)(k - 1 - straight) * 40 * Lr;
-        }
+        });
+        N.max_nodes = *std::max_element(mxn.begin(), mxn.end());
+        for (int64_t i = 0; i < n; i++) N.h_route_off[i + 1] += N.h_route_off[i];
+        const int64_t npt = d->pt_off ? d->pt_off[n] : 0, nhs = d->hs_off ? d->hs_off[n] : 0,
+                      nvs = d->vs_off ? d->vs_off[n] : 0;
+        auto check_cells = [&](const int* a, int64_t cnt, const char* what) {
+            igr::parallel_ranges(cnt, nth, [&](int64_t a0, int64_t a1, int) {
+                for (int64_t i = a0; i < a1; i++)
+                    if (a[i] < 0 || a[i] >= XY) throw std::invalid_argument(what);
+            });
+        };
+        check_cells(d->pts, npt, "scheduler point outside the grid");
+        check_cells(d->hs, 2 * nhs, "segment outside the grid");
+        check_cells(d->vs, 2 * nvs, "segment outside the grid");
         hipStream_t s = c->stream;
         N.node_off.upload(d->node_off, n + 1, s);
         N.node.upload(d->nodes, N.h_node_off[n], s);
         N.parent.upload(d->parents, N.h_node_off[n], s);
         N.route_off.upload(N.h_route_off.data(), n + 1, s);
-        // Every route starts empty: r[0] = 1.
-        std::vector<int> init(N.h_route_off[n], 0);
-        for (int64_t i = 0; i < n; i++) init[N.h_route_off[i]] = 1;
-        N.routes.upload(init.data(), init.size(), s);
+        N.routes.zero(N.h_route_off[n], s);
+        if (n > 0) hipLaunchKernelGGL(k_route_starts, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N.route_off.p, n,
+                                      N.routes.p);
+        HIP_OK(hipGetLastError());
         N.h_rbuf.assign(n, -1);
         N.clear_dag_routes();
         N.h_rptr.resize(n);
@@ -247,14 +273,6 @@ int igr_set_nets(igr_ctx* c, const igr_net_desc* d) {
         N.pt_off.upload(d->pt_off ? d->pt_off : z.data(), n + 1, s);
         N.hs_off.upload(d->hs_off ? d->hs_off : z.data(), n + 1, s);
         N.vs_off.upload(d->vs_off ? d->vs_off : z.data(), n + 1, s);
-        int64_t npt = d->pt_off ? d->pt_off[n] : 0, nhs = d->hs_off ? d->hs_off[n] : 0,
-                nvs = d->vs_off ? d->vs_off[n] : 0;
-        for (int64_t i = 0; i < npt; i++)
-            if (d->pts[i] < 0 || d->pts[i] >= XY) throw std::invalid_argument("scheduler point outside the grid");
-        for (int64_t i = 0; i < 2 * nhs; i++)
-            if (d->hs[i] < 0 || d->hs[i] >= XY) throw std::invalid_argument("segment outside the grid");
-        for (int64_t i = 0; i < 2 * nvs; i++)
-            if (d->vs[i] < 0 || d->vs[i] >= XY) throw std::invalid_argument("segment outside the grid");
         N.pt.upload(d->pts, npt, s);
         N.hs.upload(d->hs, 2 * nhs, s);
         N.vs.upload(d->vs, 2 * nvs, s);
