@@ -110,14 +110,14 @@ int igr_run(igr_ctx* c, igrh_design* h, const igr_run_opts* o, igr_run_stats* st
                     x.err = igrh_error();
                     return IGR_EINVAL;
                 }
-                const igr::DagSet& S = *igrh_dags_ref(x.h);
-                d->node_off = S.node_off.data();
-                d->nodes = S.node.data();
-                d->par_off = S.par_off.data();
-                d->par = S.par.data();
-                d->slot = S.slot.data();
-                d->child_num = S.child_num.data();
-                d->depth = S.depth.data();
+                const igr::DagSet& ds = *igrh_dags_ref(x.h);
+                d->node_off = ds.node_off.data();
+                d->nodes = ds.node.data();
+                d->par_off = ds.par_off.data();
+                d->par = ds.par.data();
+                d->slot = ds.slot.data();
+                d->child_num = ds.child_num.data();
+                d->depth = ds.depth.data();
                 x.secs = now_s() - t;
                 return IGR_OK;
             };
