@@ -101,7 +101,8 @@ int igr_pattern_route(igr_ctx* ctx, const int* order, int64_t n, int form, igr_p
  * has parents par[par_off[i] .. par_off[i+1]) (indices local to the net,
  * par_off global over all nodes) and fills child slot slot[k] of parent
  * par[k]; child_num[i] is node i's slot count and depth[i] its longest-path
- * depth.  pts are the scheduler points of each net. */
+ * depth.  pts are the scheduler points of each net.  node_off, par_off and
+ * pt_off start at 0. */
 typedef struct igr_dag_desc {
     int64_t n;
     const int64_t* node_off;

@@ -557,6 +557,7 @@ void dag_pass_batched(igr_ctx* c, const int* nets, int64_t n, int mode, const ig
     igr_dag_desc dgv = *dg_in;
     const igr_dag_desc* dg = &dgv;
     // The scheduler's points are checked here, the DAG structure on the device.
+    if (dg->pt_off[0] != 0) throw std::invalid_argument("DAG point offsets must start at 0");
     std::atomic<int> bad_pt{0};
     pfor(n, [&](int64_t a0, int64_t a1) {
         int bad = 0;
@@ -867,6 +868,7 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
         }
         const int XY = c->g.X * c->g.Y, L = c->g.L, LL = L * L, ML = L + 1;
         // Validate the DAGs before anything reaches the device.
+        if (dg->node_off[0] != 0 || dg->pt_off[0] != 0) throw std::invalid_argument("DAG offsets must start at 0");
         for (int64_t i = 0; i < n; i++) {
             const int64_t b = dg->node_off[i], e = dg->node_off[i + 1];
             if (e <= b) throw std::invalid_argument("empty DAG");
