@@ -868,7 +868,8 @@ int igr_dag_pass(igr_ctx* c, const int* nets, int64_t n, int mode, int form, con
         }
         const int XY = c->g.X * c->g.Y, L = c->g.L, LL = L * L, ML = L + 1;
         // Validate the DAGs before anything reaches the device.
-        if (dg->node_off[0] != 0 || dg->pt_off[0] != 0) throw std::invalid_argument("DAG offsets must start at 0");
+        if (dg->node_off[0] != 0 || dg->par_off[0] != 0 || dg->pt_off[0] != 0)
+            throw std::invalid_argument("DAG offsets must start at 0");
         for (int64_t i = 0; i < n; i++) {
             const int64_t b = dg->node_off[i], e = dg->node_off[i + 1];
             if (e <= b) throw std::invalid_argument("empty DAG");
