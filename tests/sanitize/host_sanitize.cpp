@@ -34,11 +34,26 @@ int main(int argc, char** argv) {
             }
         std::vector<int> nets(gi.subnets);
         for (int64_t i = 0; i < gi.subnets; i++) nets[i] = (int)i;
-        for (int mode = 0; mode < 3; mode++)
+        for (int mode = 0; mode < 3; mode++) {
             if (igrh_build_dags(h, nets.data(), (int64_t)nets.size(), xs.data(), ys.data(), mode, 4)) {
                 printf("dags: %s\n", igrh_error());
                 return 1;
             }
+            // the same DAGs in two steps, points then structure
+            if (igrh_build_dags_part(h, nets.data(), (int64_t)nets.size(), xs.data(), ys.data(), mode, 3,
+                                     IGRH_DAG_POINTS) ||
+                igrh_build_dags_part(h, nets.data(), (int64_t)nets.size(), xs.data(), ys.data(), mode, 2,
+                                     IGRH_DAG_STRUCTURE)) {
+                printf("dags in two steps: %s\n", igrh_error());
+                return 1;
+            }
+        }
+        // a structure build that does not follow the points of as many nets
+        if (!igrh_build_dags_part(h, nets.data(), (int64_t)nets.size() - 1, xs.data(), ys.data(), 0, 2,
+                                  IGRH_DAG_STRUCTURE)) {
+            printf("structure build without its points accepted\n");
+            return 1;
+        }
         std::vector<int64_t> off(gi.subnets + 1);
         std::vector<int> data;
         for (int64_t i = 0; i < gi.subnets; i++) {
