@@ -17,29 +17,6 @@ struct DagNet {
     std::vector<int> node, par_off, par, slot, child_num, depth, points;
 };
 
-// Allocator that leaves elements default-initialised on resize (no zero
-// fill): the DAG arrays are overwritten in full right after they are sized.
-template <class T>
-struct NoInitAlloc : std::allocator<T> {
-    template <class U>
-    struct rebind {
-        using other = NoInitAlloc<U>;
-    };
-    NoInitAlloc() = default;
-    template <class U>
-    NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
-    template <class U>
-    void construct(U* p) noexcept {
-        ::new (static_cast<void*>(p)) U;
-    }
-    template <class U, class... A>
-    void construct(U* p, A&&... a) {
-        ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
-    }
-};
-template <class T>
-using RawVec = std::vector<T, NoInitAlloc<T>>;
-
 // Consecutive nets' DAGs as one builder thread produces them (par local,
 // npar = parent count per node).
 struct DagChunk {

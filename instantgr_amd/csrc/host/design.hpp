@@ -4,10 +4,37 @@
 // Steiner trees.  Pure C++17; no GPU code here.
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace igr {
+
+// Allocator that leaves elements default-initialised on resize (no zero
+// fill), for arrays that are written in full right after they are sized (the
+// pattern inputs, the detour DAGs): their pages are then first touched by the
+// parallel fill instead of a serial zeroing pass.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <class U>
+    NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+    template <class U>
+    void construct(U* p) noexcept {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template <class U, class... A>
+    void construct(U* p, A&&... a) {
+        ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+    }
+};
+template <class T>
+using RawVec = std::vector<T, NoInitAlloc<T>>;
 
 // ---------------------------------------------------------------------------
 // Parsed design in metal-layer coordinates.  A 3-D position is encoded as

@@ -8,15 +8,17 @@ namespace igr {
 struct PatternInputs {
     int64_t n = 0;
     std::vector<int64_t> node_off;   // per subnet
-    std::vector<int> node;           // l*X*Y + x*Y + y (l = L for Steiner), DFS pre-order
-    std::vector<int> parent;         // local index of the parent (-1 for the root)
+    // written in full after sizing (every node by the DFS, every footprint
+    // entry by the fill pass, every position by the counting sort)
+    RawVec<int> node;                // l*X*Y + x*Y + y (l = L for Steiner), DFS pre-order
+    RawVec<int> parent;              // local index of the parent (-1 for the root)
     std::vector<int64_t> pt_off;     // scheduler points (x*Y+y)
-    std::vector<int> pt;
+    RawVec<int> pt;
     std::vector<int64_t> hs_off;     // horizontal segments, pairs of x*Y+y
-    std::vector<int> hs;
+    RawVec<int> hs;
     std::vector<int64_t> vs_off;     // vertical segments
-    std::vector<int> vs;
-    std::vector<int> order;          // subnets by hpwl, largest first (ties: lower id first)
+    RawVec<int> vs;
+    RawVec<int> order;               // subnets by hpwl, largest first (ties: lower id first)
     int max_nodes = 0;
 };
 
