@@ -52,7 +52,8 @@ void adj_from_edges(int n, const std::vector<std::pair<int, int>>& edges, Adj& a
     }
     for (int i = 0; i < n; i++) a.off[i + 1] += a.off[i];
     a.nb.resize(a.off[n]);
-    std::vector<int> cur(a.off.begin(), a.off.end() - 1);
+    thread_local std::vector<int> cur;
+    cur.assign(a.off.begin(), a.off.end() - 1);
     for (auto& e : edges) {
         a.nb[cur[e.first]++] = e.second;
         a.nb[cur[e.second]++] = e.first;
@@ -602,7 +603,17 @@ void split_net(const Coords& C, const Grid& g, const std::vector<int>& pins, std
         par[u] = v;
         sz[v] += sz[u];
     }
-    std::vector<std::vector<int>> members(n), net_edge(n);
+    // Member and edge lists per root, kept per thread between nets so the
+    // inner lists keep their capacity; emptied here for this net.
+    thread_local std::vector<std::vector<int>> members, net_edge;
+    if (members.size() < (size_t)n) {
+        members.resize(n);
+        net_edge.resize(n);
+    }
+    for (int i = 0; i < n; i++) {
+        members[i].clear();
+        net_edge[i].clear();
+    }
     for (int i = 0; i < n; i++) members[find(i)].push_back(i);
     for (int i = 0; i < (int)edges.size(); i++) {
         int u = std::get<0>(edges[i]), v = std::get<1>(edges[i]);
@@ -623,12 +634,16 @@ void split_net(const Coords& C, const Grid& g, const std::vector<int>& pins, std
     for (int r = 0; r < n; r++) {
         if (find(r) != r || members[r].empty()) continue;
         SubOut s;
+        s.node.reserve(members[r].size());
+        s.pins.reserve(members[r].size());
         for (int m : members[r]) {
             local[m] = (int)s.node.size();
             s.node.push_back(whole.node[m]);
             if (whole.node[m] < g.L * XY) s.pins.push_back(whole.node[m]);
         }
-        Dsu up((int)s.node.size());
+        thread_local Dsu up(0);
+        up.p.resize(s.node.size());
+        std::iota(up.p.begin(), up.p.end(), 0);
         acc.clear();
         for (int ei : net_edge[r]) {
             int a = local[std::get<0>(edges[ei])], b = local[std::get<1>(edges[ei])];
